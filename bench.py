@@ -1,0 +1,232 @@
+"""Benchmark of the streaming hot path on one MI355X.
+
+Default workload (one GPU): the full FP64 streaming suite --
+BS1-BS5 on N = 2^27 doubles (U[-1,1), LCG seed 1, alpha = 0.37, beta = -1.21),
+BS6-BS8 on a 64^3-element hex box at degree 7 (134,217,728 local nodes,
+90,518,849 global nodes, same seed), BS0 as 10^4 empty launches, plain and
+replayed from a HIP graph.
+
+One timed "step" is one BS8 gather+scatter (ogs_t::GatherScatter(q, 1, Add,
+Sym)) over the whole mesh; the headline value is its algorithmic HBM
+bandwidth, 4(Ng+1) + 4Nl + 16Nl bytes per call (bs8Run.cpp:56-63), over the
+host-timed K steps.  Every other kernel is timed the same way and reported
+under "suite".  The roofline object uses the BS8 kernel's average duration
+from HIP events recorded on the library's stream.  The CPU baseline times
+the oracle's restatement of the reference's Serial mode on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SPEC_HBM_GBS = 8000.0
+METRIC = "HBM GB/s (% peak) + GDOF/s per BS kernel; mixed/batched vs plain FP64"
+PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "bs8_gather_scatter_64cube_p7_counters.md")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup):
+    """(host seconds per call, event seconds per call) over `steps` calls after `warmup`."""
+    from streamparanumal_amd import device
+    for _ in range(warmup):
+        fn()
+    device.finish()
+    e0, e1 = device.Event(), device.Event()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    device.finish()
+    host = (time.perf_counter() - t0) / steps
+    return host, e0.elapsed_ms(e1) * 1e-3 / steps
+
+
+def read_traffic():
+    """HBM bytes per BS8 launch from the committed rocprofv3 counter summary, if present."""
+    if not os.path.exists(PROFILE_COUNTERS):
+        return None
+    text = open(PROFILE_COUNTERS).read()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline():
+    """The oracle's Serial restatement of BS8 on a 32^3 p=7 box, 1 host thread."""
+    from oracle import pyoracle
+    pyoracle.lib()
+    nx, p, reps = 32, 7, 30
+    r = pyoracle.time_bs8(nx, p, reps, 1)
+    nl, ng = r["nlocal"], r["nglobal"]
+    bytes_ = 4 * (ng + 1) + 4 * nl + 16 * nl
+    return {"value": round(bytes_ / r["seconds"] / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"BS8 gather+scatter FP64, {nx}^3 hex p={p} ({nl} local / {ng} global nodes), "
+                      f"{reps} calls of the oracle's Serial-mode restatement (gcc -O3), 1 thread; "
+                      f"{r['seconds'] * 1e3:.2f} ms/call, {ng / r['seconds'] / 1e9:.4f} GDOF/s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=64, help="hex elements per side for BS6-BS8")
+    ap.add_argument("--degree", type=int, default=7)
+    ap.add_argument("--log2n", type=int, default=27, help="BS1-BS5 vector length 2^log2n")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    K, W = args.steps, args.warmup
+
+    import numpy as np
+
+    import streamparanumal_amd as spx
+    from streamparanumal_amd import device, ogs, streaming
+    from streamparanumal_amd.device import DeviceArray
+
+    ogs.platform_open(0)
+    props = device.props()
+    log("device", props["name"], props["arch"], props["compute_units"], "CUs")
+
+    # ---------------------------------------------------------------- BS6-BS8
+    nx, p = args.nx, args.degree
+    t0 = time.perf_counter()
+    ids = ogs.hex_ids(nx, nx, nx, p)
+    g = ogs.Ogs(ids)
+    del ids
+    Nl, Ng = g.N, g.NgatherGlobal
+    log(f"mesh {nx}^3 p={p}: Nl={Nl} Ng={Ng}, setup {time.perf_counter() - t0:.1f}s "
+        f"(ogs {g.info['setup_seconds']:.1f}s), blocks {g.info['NblocksT']} x <= {g.info['block_nnz']}")
+    q = DeviceArray(Nl)
+    device.fill_uniform(q, 1, 0)
+    gq = DeviceArray(Ng)
+
+    bytes_bs8 = 4 * (Ng + 1) + 4 * Nl + 16 * Nl
+    bytes_bs6 = 4 * (Ng + 1) + 4 * Nl + 8 * Nl + 8 * Ng
+    bytes_bs7 = 4 * (Ng + 1) + 4 * Nl + 8 * Ng + 8 * Nl
+
+    host8, ev8 = timed(lambda: g.gather_scatter(q), K, W)
+    device.check_kernel_errors()
+    log(f"BS8 host {host8 * 1e3:.3f} ms/call, kernel {ev8 * 1e3:.3f} ms -> {bytes_bs8 / ev8 / 1e9:.1f} GB/s")
+    suite = {}
+
+    def entry(name, bytes_, dofs, host, ev, extra=None):
+        d = {"bytes_per_call": int(bytes_), "host_ms": round(host * 1e3, 4), "kernel_ms": round(ev * 1e3, 4),
+             "GBs": round(bytes_ / ev / 1e9, 1), "GBs_host": round(bytes_ / host / 1e9, 1),
+             "pct_spec_peak": round(100 * bytes_ / ev / 1e9 / SPEC_HBM_GBS, 1),
+             "GDOFs": round(dofs / ev / 1e9, 3)}
+        if extra:
+            d.update(extra)
+        suite[name] = d
+        log(name, json.dumps(d))
+
+    entry("BS8_gather_scatter", bytes_bs8, Ng, host8, ev8)
+    h, e = timed(lambda: g.gather(gq, q, 1, spx.ADD, spx.TRANS), K, W)
+    entry("BS6_gather", bytes_bs6, Ng, h, e)
+    h, e = timed(lambda: g.scatter(q, gq, 1, spx.NOTRANS), K, W)
+    entry("BS7_scatter", bytes_bs7, Ng, h, e)
+    device.check_kernel_errors()
+    del q, gq, g
+
+    # ---------------------------------------------------------------- BS1-BS5
+    n = 1 << args.log2n
+    a, b = DeviceArray(n), DeviceArray(n)
+    device.fill_uniform(a, 1, 0)
+    device.fill_uniform(b, 1, n)
+    h, e = timed(lambda: streaming.copy(a, b), K, W)
+    entry("BS1_copy", 16 * n, n, h, e)
+    measured_peak = 16 * n / e / 1e9
+    h, e = timed(lambda: streaming.axpy(0.37, a, -1.21, b), K, W)
+    entry("BS2_axpy", 24 * n, n, h, e)
+    for mode, tag in ((spx.REDUCE_REFERENCE, "reference"), (spx.REDUCE_FAST, "fast")):
+        red = streaming.Reducer(n, mode, 256)
+        h, e = timed(lambda: red.norm(a), K, W)
+        entry(f"BS3_norm_{tag}", 8 * n, n, h, e)
+        h, e = timed(lambda: red.dot(a, b), K, W)
+        entry(f"BS4_dot_{tag}", 16 * n, n, h, e)
+        h, e = timed(lambda: red.cg_update(a, 1e-9, b), K, W)
+        entry(f"BS5_cg_update_{tag}", 24 * n, n, h, e)
+    pvec, xvec = DeviceArray(n), DeviceArray(n)
+    device.fill_uniform(pvec, 1, 2 * n)
+    device.fill_uniform(xvec, 1, 3 * n)
+    red = streaming.Reducer(n, spx.REDUCE_FAST, 256)
+    h, e = timed(lambda: red.cg_full(1e-9, pvec, a, xvec, b), K, W)
+    entry("BS5_cg_full_fast", 48 * n, n, h, e)
+    del pvec, xvec, a, b
+    for name, d in suite.items():
+        d["pct_measured_peak"] = round(100 * d["GBs"] / measured_peak, 1)
+
+    # ---------------------------------------------------------------- BS0
+    ta, tb = DeviceArray(1), DeviceArray(1)
+    nl = 10000
+    for _ in range(100):
+        streaming.bs0(1, ta, tb)
+    device.finish()
+    t0 = time.perf_counter()
+    for _ in range(nl):
+        streaming.bs0(1, ta, tb)
+    device.finish()
+    plain = (time.perf_counter() - t0) / nl
+    graph = device.Graph.capture(lambda: [streaming.bs0(1, ta, tb) for _ in range(nl)])
+    graph.launch()
+    device.finish()
+    t0 = time.perf_counter()
+    graph.launch()
+    device.finish()
+    replay = (time.perf_counter() - t0) / nl
+    suite["BS0_launch"] = {"launches": nl, "plain_us_per_launch": round(plain * 1e6, 3),
+                           "graph_us_per_launch": round(replay * 1e6, 3)}
+    log("BS0", json.dumps(suite["BS0_launch"]))
+
+    # ---------------------------------------------------------------- report
+    value = bytes_bs8 / host8 / 1e9
+    achieved = bytes_bs8 / ev8 / 1e9
+    traffic = read_traffic()
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, 1 thread) ...")
+        cpu = cpu_baseline()
+        log("cpu baseline", json.dumps(cpu))
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(host8 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: U[-1,1) from the reference LCG (seed 1); lexicographic GLL ids",
+        "config": {"workload": f"BS8 gather+scatter FP64 (Add, Sym, K=1) on a {nx}^3 hex box at p={p} "
+                               f"({Nl} local / {Ng} global nodes); suite BS1-BS5 at N=2^{args.log2n}",
+                   "nx": nx, "degree": p, "local_nodes": Nl, "global_nodes": Ng, "vector_n": n,
+                   "bytes_per_step": int(bytes_bs8), "GDOFs": round(Ng / host8 / 1e9, 3),
+                   "pct_spec_peak": round(100 * value / SPEC_HBM_GBS, 1),
+                   "pct_measured_peak": round(100 * achieved / measured_peak, 1),
+                   "measured_peak_GBs": round(measured_peak, 1), "device": props["name"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": SPEC_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / SPEC_HBM_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "suite": suite,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,82 @@
+/*
+ * spx_host.h -- C ABI of libspxhost.so, the host layer above libspx.
+ *
+ * libspxhost.so holds the C++ mirror of the reference's libs/core and
+ * libs/ogs surface (platform_t, memory, settings_t, ogs_t, mesh_t).  C++
+ * callers use those classes directly (streamparanumal_amd/libp/include);
+ * this header exports the ogs_t and mesh pieces to non-C++ callers (the
+ * Python package, ctypes).  Every function returns SPX_OK or a negative
+ * SPX_ERR_* code with the message in spxh_last_error().
+ */
+#ifndef SPX_HOST_H
+#define SPX_HOST_H
+
+#include <stdint.h>
+
+#include "spx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spxh_ogs_s *spxh_ogs_t;
+
+typedef struct {
+  int32_t N;              /* local entries */
+  int32_t Ngather;        /* positive gathered nodes (ogsBase.hpp:46) */
+  int32_t NlocalT;        /* gathered nodes, all groups (ogsBase.hpp:48) */
+  int32_t NlocalP;        /* gathered nodes with a positive member (ogsBase.hpp:50) */
+  int32_t nnzT, nnzN;     /* CSR entries (ogsOperator.hpp:51-52) */
+  int32_t NblocksT, NblocksN;
+  int32_t block_nnz;      /* LDS tile: max entries per row block */
+  int32_t gather_defined; /* ogsBase.hpp:57 */
+  int32_t kind, unique;
+  int64_t NgatherGlobal;
+  double setup_seconds;
+} spxh_ogs_info_t;
+
+const char *spxh_last_error(void);
+
+/* platform_t on device `device_id` in HIP mode (libs/core/platformDeviceConfig.cpp) */
+int spxh_platform_open(int device_id);
+
+/* ogs_t::Setup(N, ids, comm, kind, Auto, unique, verbose, platform)
+   (libs/ogs/ogsSetup.cpp:62-71).  ids (n entries) are updated in place when
+   unique=1, as the reference writes the flags back (ogsSetup.cpp:149-151). */
+int spxh_ogs_setup(int32_t n, int64_t *ids, int kind, int unique, int verbose, spxh_ogs_t *out);
+int spxh_ogs_destroy(spxh_ogs_t ogs);
+int spxh_ogs_info(spxh_ogs_t ogs, spxh_ogs_info_t *info);
+/* host copies of the operator: which = 0 (T-CSR) or 1 (N-CSR); arrays sized
+   rows+1, nnz and blocks+1 (any may be NULL) */
+int spxh_ogs_copy_csr(spxh_ogs_t ogs, int which, int32_t *rowStarts, int32_t *colIds, int32_t *blockStarts);
+/* re-cut the row blocks for another LDS tile (autotuner) */
+int spxh_ogs_set_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
+/* ogs_t::SetupGlobalToLocalMapping (ogsSetup.cpp:833-849) */
+int spxh_ogs_global_to_local(spxh_ogs_t ogs, int32_t *g2l);
+
+/* ogs_t device operations on device pointers, enqueued on the current stream
+   (ogs.cpp:39-125, 205-301, 394-492) */
+int spxh_ogs_gather_scatter(spxh_ogs_t ogs, int type, int op, int trans, int k, void *v);
+int spxh_ogs_gather(spxh_ogs_t ogs, int type, int op, int trans, int k, const void *v, void *gv);
+int spxh_ogs_scatter(spxh_ogs_t ogs, int type, int trans, int k, const void *gv, void *v);
+
+/* The host half of the setup alone (no device needed): grouping, unique
+   flagging (written back into ids), row numbering, T- and N-CSR. */
+typedef struct spxh_csr_s *spxh_csr_t;
+int spxh_csr_build(int32_t n, int64_t *ids, int kind, int unique, spxh_csr_t *out);
+/* sizes[0..4] = NrowsT, NrowsN, nnzT, nnzN, gather_defined */
+int spxh_csr_sizes(spxh_csr_t csr, int32_t *sizes);
+int spxh_csr_copy(spxh_csr_t csr, int which, int32_t *rowStarts, int32_t *colIds);
+int spxh_csr_destroy(spxh_csr_t csr);
+/* row blocks of at most cap entries (greedy); blockStarts holds nrows+2 */
+int spxh_block_rows(int32_t nrows, const int32_t *rowStarts, int32_t cap, int32_t *nblocks,
+                    int32_t *blockStarts);
+
+/* mesh_t hex box ids: lexicographic GLL-grid ids of an nx x ny x nz box at
+   degree p, n = nx*ny*nz*(p+1)^3 entries (meshSetupBoxHex3D.cpp:87-130) */
+int spxh_mesh_hex_ids(int nx, int ny, int nz, int p, int64_t *ids);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+// BS8: gather+scatter, ogs.GatherScatter(o_q, K, Add, Sym) (BS/BS8/src/bs8Run.cpp:29-78, bs8Main.cpp:29-57).
+// Adds --seed (inputs from the reference LCG), --fields K (the reference runs
+// K = 1) and --block-nnz (LDS tile of the row blocks, the autotuner's knob).
+#include "bsCommon.hpp"
+
+using namespace libp;
+
+int main(int argc, char **argv) {
+  return bs::run([&] {
+    settings_t settings(comm_t::world());
+    platformAddSettings(settings);
+    meshAddSettings(settings);
+    bs::addVerbose(settings);
+    bs::addSeed(settings);
+    settings.newSetting("--fields", "--fields", "FIELDS", "1", "Fields per node (K)");
+    settings.newSetting("--block-nnz", "--block-nnz", "BLOCK NNZ", "2048", "Max column entries per row block");
+    settings.parseSettings(argc, argv);
+    if (settings.helpRequested()) return;
+
+    platform_t platform(settings);
+    if (settings.compareSetting("VERBOSE", "TRUE")) {
+      std::cout << "Settings:\n\n";
+      platformReportSettings(settings);
+      meshReportSettings(settings);
+    }
+    mesh_t mesh(platform, settings, settings.comm);
+    int K = 1, blockNnz = 2048;
+    unsigned long long seed = 1;
+    settings.getSetting("FIELDS", K);
+    settings.getSetting("BLOCK NNZ", blockNnz);
+    settings.getSetting("SEED", seed);
+    LIBP_ABORT("--fields must be >= 1", K < 1);
+    if (blockNnz != mesh.ogs.localOperator()->blockNnz) mesh.ogs.localOperator()->SetBlockNnz(blockNnz);
+
+    const dlong N = mesh.Np * mesh.Nelements;
+    deviceMemory<dfloat> o_q = platform.malloc<dfloat>((size_t)N * K);
+    bs::fillUniform(o_q, seed, 0);
+    for (int n = 0; n < 5; ++n) { mesh.ogs.GatherScatter(o_q, K, ogs::Add, ogs::Sym); }
+
+    const int Ntests = 50;
+    deviceTimer_t dt;
+    timePoint_t start = GlobalPlatformTime(platform);
+    dt.start();
+    for (int n = 0; n < Ntests; ++n) { mesh.ogs.GatherScatter(o_q, K, ogs::Add, ogs::Sym); }
+    dt.stop();
+    timePoint_t end = GlobalPlatformTime(platform);
+    const double elapsedTime = ElapsedTime(start, end) / Ntests;
+    const double kernelTime = dt.seconds() / Ntests;
+    SPX_CHECK(spx_check_kernel_errors());
+
+    hlong NtotalGlobal = (hlong)mesh.Nelements * mesh.Np;
+    mesh.comm.Allreduce(NtotalGlobal);
+    const hlong NgatherGlobal = mesh.ogs.NgatherGlobal;
+    size_t bytesIn = 0, bytesOut = 0;
+    bytesIn += (NgatherGlobal + 1) * sizeof(dlong);  // row starts
+    bytesIn += NtotalGlobal * sizeof(dlong);         // local ids
+    bytesIn += (size_t)K * NtotalGlobal * sizeof(dfloat);
+    bytesOut += (size_t)K * NtotalGlobal * sizeof(dfloat);
+    const size_t bytes = bytesIn + bytesOut;
+    const hlong Ndofs = NgatherGlobal;
+    const double Nflops = (double)K * NtotalGlobal;
+    if (mesh.rank == 0) {
+      printf("BS8 = [%d, " hlongFormat ", %5.4le, %5.4le, %6.2f, %6.2f]; %% GatherScatter [N, DOFs, elapsed, DOFs/(ranks*s), avg BW (GB/s), avg GFLOPs] \n",
+             mesh.N, Ndofs, elapsedTime, Ndofs / (mesh.size * elapsedTime), bytes / (1.0e9 * elapsedTime),
+             Nflops / (1.0e9 * elapsedTime));
+      printf("BS8device = [%d, " hlongFormat ", %5.4le, %5.4le, %6.2f]; %% GatherScatter kernel-only [N, DOFs, elapsed, DOFs/s, BW (GB/s)], K=%d, block nnz %d\n",
+             mesh.N, Ndofs, kernelTime, Ndofs / kernelTime, bytes / (1.0e9 * kernelTime), K,
+             (int)mesh.ogs.localOperator()->blockNnz);
+    }
+  });
+}

@@ -1,0 +1,83 @@
+// Shared pieces of the BS0-BS8 drivers (BS/BSn/bsNMain.cpp, src/bsN*.cpp).
+//
+// Each driver keeps the reference's flags, defaults, timing loop and printed
+// MATLAB-style row.  Additions are long-only flags (no short key of the
+// reference is reused): --seed initialises the inputs with the reference LCG
+// (the reference leaves them uninitialised), and a second printed row gives
+// the kernel-only time from HIP events next to the reference's host timing.
+#pragma once
+
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+
+#include "mesh.hpp"
+#include "ogs.hpp"
+#include "platform.hpp"
+#include "settings.hpp"
+#include "timer.hpp"
+
+namespace bs {
+
+using namespace libp;
+
+// Runs a driver body, turning libp::exception into a message and exit code 1.
+inline int run(const std::function<void()> &body) {
+  try {
+    body();
+    return 0;
+  } catch (const libp::exception &e) {
+    std::fprintf(stderr, "%s", e.what());
+    return 1;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+}
+
+inline void addVerbose(settings_t &s) {
+  s.newSetting("-v", "--verbose", "VERBOSE", "FALSE", "Enable verbose output", {"TRUE", "FALSE"});
+}
+
+inline void addSeed(settings_t &s, const char *dflt = "1") {
+  s.newSetting("--seed", "--seed", "SEED", dflt, "Seed of the LCG that fills the input vectors (U[-1,1))");
+}
+
+// byte-sweep settings of BS1-BS5 (bs1Settings.cpp:35-53)
+inline void addByteSettings(settings_t &s) {
+  s.newSetting("-b", "--bytes", "BYTES", "0", "Array size in bytes");
+  s.newSetting("-bmin", "--bmin", "BMIN", "0", "Minimum array size in bytes");
+  s.newSetting("-bmax", "--bmax", "BMAX", "0", "Maximum array size in bytes");
+  s.newSetting("-s", "--bstep", "BSTEP", "1048576", "Array step size in bytes");
+}
+
+struct Sweep {
+  bool single = false;
+  long long Nmin = 0, Nmax = 0, Nstep = 1;
+};
+
+// bs1Run.cpp:32-47: default single test of 1 GB; sizes in entries = bytes/sc
+inline Sweep byteSweep(settings_t &s, long long sc) {
+  size_t B = 0, Bmin = 0, Bmax = 0, Bstep = 0;
+  s.getSetting("BYTES", B);
+  s.getSetting("BMIN", Bmin);
+  s.getSetting("BMAX", Bmax);
+  s.getSetting("BSTEP", Bstep);
+  if (!(B | Bmin | Bmax)) B = 1073741824;
+  if (B) Bmax = B;
+  Sweep w;
+  w.Nmin = (long long)(Bmin / sc);
+  w.Nmax = (long long)(Bmax / sc);
+  w.Nstep = ((long long)(Bstep / sc) > 0) ? (long long)(Bstep / sc) : 1;
+  if (B) {
+    w.single = true;
+    w.Nmin = w.Nmax = (long long)(B / sc);
+  }
+  return w;
+}
+
+inline void fillUniform(deviceMemory<dfloat> &m, unsigned long long seed, unsigned long long skip) {
+  if (m.length()) SPX_CHECK(spx_fill_uniform_f64((int64_t)m.length(), m.ptr(), seed, skip));
+}
+
+}  // namespace bs

@@ -1,0 +1,64 @@
+// Internal helpers shared by the libspx HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/spx.h"
+
+namespace spx {
+
+// ---- error state -----------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define SPX_HIP(call)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::spx::fail(SPX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Launch check: catches invalid launch configurations without waiting.
+#define SPX_LAUNCHED(name)                                                        \
+  do {                                                                            \
+    hipError_t e_ = hipGetLastError();                                            \
+    if (e_ != hipSuccess)                                                         \
+      return ::spx::fail(SPX_ERR_HIP, std::string(name) + " launch: " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- device state ------------------------------------------------------------
+bool device_ready();
+hipStream_t current_stream();
+int *kernel_error_flag();   // device word kernels set on a violated precondition
+int num_cus();
+
+#define SPX_NEED_DEVICE()                                                          \
+  do {                                                                            \
+    if (!::spx::device_ready())                                                   \
+      return ::spx::fail(SPX_ERR_NODEVICE, "no device open: call spx_device_open first"); \
+  } while (0)
+
+// ---- tuning table ----------------------------------------------------------------
+spx_tuning_t tuning_for(const char *kernel, spx_tuning_t defaults);
+
+// ---- device helpers ----------------------------------------------------------------
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with a private L2.  Remap the flat workgroup id so that XCD x runs one
+// contiguous range of tiles: neighbouring tiles (which share cache lines at
+// their edges) then meet in the same L2.  Bijective on [0, n).
+__device__ __forceinline__ int xcd_tile(int bid, int n, int nxcd) {
+  if (nxcd <= 1) return bid;
+  const int q = n / nxcd, r = n % nxcd;
+  const int x = bid % nxcd, l = bid / nxcd;
+  return (x < r) ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
+
+inline bool aligned(const void *p, size_t a) {
+  return (reinterpret_cast<uintptr_t>(p) % a) == 0;
+}
+
+}  // namespace spx

@@ -1,0 +1,569 @@
+// BS0-BS5 streaming kernels for gfx950.
+//
+// BS1/BS2 stream 16-byte (dwordx4) loads and stores, several per thread, so a
+// wave moves 1 KiB per instruction and keeps many requests in flight.
+// BS3-BS5 reduce in one of two orders:
+//   REFERENCE - the exact loop nest of bs3.okl/bs4.okl/bs5.okl: Nblock =
+//               min(ceil(N/bs), bs) blocks, thread t of block b sums
+//               a[t + b*bs + j*bs*Nblock] for j = 0,1,.. in order, a halving
+//               LDS tree down to s[0]+s[1], and a second one-block pass.  The
+//               serial chain per thread is kept; only its loads are issued
+//               ahead (U independent loads, then U dependent adds).
+//   FAST      - a fixed grid over the whole machine, compensated (Neumaier)
+//               per-thread sums, a fixed tree, and a one-block second pass.
+//               Deterministic run to run; within 1e-14 of the reference order.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "spx_internal.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace spx;
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ dbl2 ld2(const dbl2 *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st2(dbl2 *p, dbl2 v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// ---------------------------------------------------------------------- BS0 --
+// bs0.okl:27-39: one work-item; the store never happens for N != 0.
+__global__ void k_bs0(int n, const double *a, double *b) {
+  (void)a;
+  if ((int)threadIdx.x == n && (int)blockIdx.x == n) b[0] = 0;
+}
+
+// ---------------------------------------------------------------------- BS1 --
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void k_copy2(int64_t n2, const dbl2 *__restrict__ a,
+                                               dbl2 *__restrict__ b) {
+  const int64_t base = (int64_t)blockIdx.x * (U * 256) + threadIdx.x;
+  dbl2 r[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = base + u * 256;
+    if (i < n2) r[u] = ld2<NT>(a + i);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = base + u * 256;
+    if (i < n2) st2<NT>(b + i, r[u]);
+  }
+}
+
+// scalar path for unaligned pointers and the odd tail element
+__global__ __launch_bounds__(256) void k_copy1(int64_t lo, int64_t n, const double *__restrict__ a,
+                                               double *__restrict__ b) {
+  int64_t i = lo + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * 256;
+  for (; i < n; i += st) b[i] = a[i];
+}
+
+// ---------------------------------------------------------------------- BS2 --
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void k_axpy2(int64_t n2, double alpha, const dbl2 *__restrict__ x,
+                                               double beta, dbl2 *__restrict__ y) {
+  const int64_t base = (int64_t)blockIdx.x * (U * 256) + threadIdx.x;
+  dbl2 rx[U], ry[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = base + u * 256;
+    if (i < n2) {
+      rx[u] = ld2<NT>(x + i);
+      ry[u] = ld2<NT>(y + i);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = base + u * 256;
+    if (i < n2) {
+      dbl2 o;
+      o.x = alpha * rx[u].x + beta * ry[u].x;
+      o.y = alpha * rx[u].y + beta * ry[u].y;
+      st2<NT>(y + i, o);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_axpy1(int64_t lo, int64_t n, double alpha,
+                                               const double *__restrict__ x, double beta,
+                                               double *__restrict__ y) {
+  int64_t i = lo + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * 256;
+  for (; i < n; i += st) y[i] = alpha * x[i] + beta * y[i];
+}
+
+// --------------------------------------------------------------- reductions --
+enum RedKind { RED_NORM = 0, RED_DOT = 1, RED_CG = 2, RED_CGFULL = 3 };
+
+struct RedArgs {
+  const double *a;   // NORM: a; DOT: a; CG: Ap; CGFULL: p
+  const double *b;   // DOT: b; CGFULL: Ap
+  double *r;         // CG, CGFULL: r (updated)
+  double *x;         // CGFULL: x (updated)
+  double alpha;
+};
+
+// One term of the reduction at element i, applying the update when the
+// kernel has one.  Loads of term i are independent of the running sum.
+template <int KIND>
+__device__ __forceinline__ double red_term(const RedArgs &A, int64_t i) {
+  if constexpr (KIND == RED_NORM) {
+    const double v = A.a[i];
+    return v * v;
+  } else if constexpr (KIND == RED_DOT) {
+    return A.a[i] * A.b[i];
+  } else if constexpr (KIND == RED_CG) {
+    const double rr = A.r[i] + A.alpha * A.a[i];
+    A.r[i] = rr;
+    return rr * rr;
+  } else {
+    A.x[i] = A.x[i] + A.alpha * A.a[i];
+    const double rr = A.r[i] - A.alpha * A.b[i];
+    A.r[i] = rr;
+    return rr * rr;
+  }
+}
+
+// halving tree of bs3.okl:48-63 on BS doubles in LDS, result s[0]+s[1]
+template <int BS>
+__device__ __forceinline__ double ref_tree(double *s) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int w = BS / 2; w >= 2; w >>= 1) {
+    if (t < w) s[t] += s[t + w];
+    __syncthreads();
+  }
+  return s[0] + s[1];
+}
+
+// REFERENCE stage 1: exactly the bsN_1 loop nest (bs3.okl:27-64).  Each
+// unrolled group first issues all of its loads, then applies the updates and
+// adds the U terms to the running sum in index order.
+template <int KIND, int U>
+__device__ __forceinline__ void ref_group(const RedArgs &A, int64_t id, int64_t stride, double &acc) {
+  double va[U], vb[U], vr[U], vx[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = id + u * stride;
+    va[u] = A.a[i];
+    if constexpr (KIND == RED_DOT || KIND == RED_CGFULL) vb[u] = A.b[i];
+    if constexpr (KIND == RED_CG || KIND == RED_CGFULL) vr[u] = A.r[i];
+    if constexpr (KIND == RED_CGFULL) vx[u] = A.x[i];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i = id + u * stride;
+    double term;
+    if constexpr (KIND == RED_NORM) {
+      term = va[u] * va[u];
+    } else if constexpr (KIND == RED_DOT) {
+      term = va[u] * vb[u];
+    } else if constexpr (KIND == RED_CG) {
+      const double rr = vr[u] + A.alpha * va[u];
+      A.r[i] = rr;
+      term = rr * rr;
+    } else {
+      A.x[i] = vx[u] + A.alpha * va[u];
+      const double rr = vr[u] - A.alpha * vb[u];
+      A.r[i] = rr;
+      term = rr * rr;
+    }
+    acc += term;
+  }
+}
+
+template <int KIND, int BS, int U>
+__global__ __launch_bounds__(BS) void k_red_ref1(int64_t n, int64_t nblock, RedArgs A,
+                                                 double *__restrict__ partials) {
+  __shared__ double s[BS];
+  const int t = threadIdx.x;
+  const int64_t stride = (int64_t)BS * nblock;
+  int64_t id = t + (int64_t)blockIdx.x * BS;
+  double acc = 0.0;
+  for (; id + (int64_t)(U - 1) * stride < n; id += (int64_t)U * stride)
+    ref_group<KIND, U>(A, id, stride, acc);
+  for (; id < n; id += stride) acc += red_term<KIND>(A, id);
+  s[t] = acc;
+  __syncthreads();
+  const double blk = ref_tree<BS>(s);
+  if (t == 0) partials[blockIdx.x] = blk;
+}
+
+// REFERENCE stage 2: bsN_2 (bs3.okl:66-102) with one block.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_red_ref2(int64_t nblock, const double *__restrict__ partials,
+                                                 double *__restrict__ out) {
+  __shared__ double s[BS];
+  const int t = threadIdx.x;
+  double r = 0.0;
+  for (int64_t id = t; id < nblock; id += (int64_t)BS * nblock) r += partials[id];
+  s[t] = r;
+  __syncthreads();
+  const double v = ref_tree<BS>(s);
+  if (t == 0) out[0] = v;
+}
+
+// FAST stage 1: grid of G blocks x 256, 16-byte loads, Neumaier sums.
+__device__ __forceinline__ void neumaier(double &sum, double &c, double v) {
+  const double t = sum + v;
+  if (fabs(sum) >= fabs(v)) c += (sum - t) + v;
+  else c += (v - t) + sum;
+  sum = t;
+}
+
+template <int KIND>
+__device__ __forceinline__ void fast_pair(const RedArgs &A, int64_t i2, double &s, double &c) {
+  const int64_t i = 2 * i2;
+  if constexpr (KIND == RED_NORM) {
+    const dbl2 v = *reinterpret_cast<const dbl2 *>(A.a + i);
+    neumaier(s, c, v.x * v.x);
+    neumaier(s, c, v.y * v.y);
+  } else if constexpr (KIND == RED_DOT) {
+    const dbl2 v = *reinterpret_cast<const dbl2 *>(A.a + i);
+    const dbl2 w = *reinterpret_cast<const dbl2 *>(A.b + i);
+    neumaier(s, c, v.x * w.x);
+    neumaier(s, c, v.y * w.y);
+  } else if constexpr (KIND == RED_CG) {
+    const dbl2 ap = *reinterpret_cast<const dbl2 *>(A.a + i);
+    dbl2 r = *reinterpret_cast<const dbl2 *>(A.r + i);
+    r.x = r.x + A.alpha * ap.x;
+    r.y = r.y + A.alpha * ap.y;
+    *reinterpret_cast<dbl2 *>(A.r + i) = r;
+    neumaier(s, c, r.x * r.x);
+    neumaier(s, c, r.y * r.y);
+  } else {
+    const dbl2 p = *reinterpret_cast<const dbl2 *>(A.a + i);
+    const dbl2 ap = *reinterpret_cast<const dbl2 *>(A.b + i);
+    dbl2 x = *reinterpret_cast<const dbl2 *>(A.x + i);
+    dbl2 r = *reinterpret_cast<const dbl2 *>(A.r + i);
+    x.x = x.x + A.alpha * p.x;
+    x.y = x.y + A.alpha * p.y;
+    r.x = r.x - A.alpha * ap.x;
+    r.y = r.y - A.alpha * ap.y;
+    *reinterpret_cast<dbl2 *>(A.x + i) = x;
+    *reinterpret_cast<dbl2 *>(A.r + i) = r;
+    neumaier(s, c, r.x * r.x);
+    neumaier(s, c, r.y * r.y);
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ void fast_one(const RedArgs &A, int64_t i, double &s, double &c) {
+  neumaier(s, c, red_term<KIND>(A, i));
+}
+
+// Block sum of (s + c) over 256 threads with a fixed shape: wave-level
+// shuffles then LDS across the 4 waves.
+__device__ __forceinline__ double block_sum256(double v) {
+  __shared__ double ws[4];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) ws[w] = v;
+  __syncthreads();
+  double out = 0.0;
+  if (threadIdx.x == 0) out = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+  return out;
+}
+
+template <int KIND, int U>
+__global__ __launch_bounds__(256) void k_red_fast1(int64_t n, RedArgs A, double *__restrict__ partials,
+                                                   int vec) {
+  double s = 0.0, c = 0.0;
+  const int64_t nthreads = (int64_t)gridDim.x * 256;
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (vec) {
+    const int64_t n2 = n / 2;
+    int64_t i2 = tid;
+    for (; i2 + (int64_t)(U - 1) * nthreads < n2; i2 += (int64_t)U * nthreads) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) fast_pair<KIND>(A, i2 + u * nthreads, s, c);
+    }
+    for (; i2 < n2; i2 += nthreads) fast_pair<KIND>(A, i2, s, c);
+    if (tid == 0 && (n & 1)) fast_one<KIND>(A, n - 1, s, c);
+  } else {
+    for (int64_t i = tid; i < n; i += nthreads) fast_one<KIND>(A, i, s, c);
+  }
+  const double b = block_sum256(s + c);
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(256) void k_red_fast2(int64_t g, const double *__restrict__ partials,
+                                                   double *__restrict__ out) {
+  double s = 0.0, c = 0.0;
+  for (int64_t i = threadIdx.x; i < g; i += 256) neumaier(s, c, partials[i]);
+  const double b = block_sum256(s + c);
+  if (threadIdx.x == 0) out[0] = b;
+}
+
+// ------------------------------------------------------------------- host ----
+
+static int vec_blocks(int64_t n2, int per_block, unsigned *out) {
+  const int64_t b = (n2 + per_block - 1) / per_block;
+  if (b > INT32_MAX) return fail(SPX_ERR_ARG, "vector too long for one launch");
+  *out = (unsigned)(b > 0 ? b : 1);
+  return SPX_OK;
+}
+
+static spx_tuning_t stream_defaults() {
+  spx_tuning_t d{};
+  d.block_threads = 256;
+  d.xcd_remap = 0;
+  d.unroll = 4;
+  d.grid_blocks = 0;
+  return d;
+}
+
+template <int U, bool NT>
+static void launch_copy2(unsigned grid, int64_t n2, const double *a, double *b) {
+  hipLaunchKernelGGL((k_copy2<U, NT>), dim3(grid), dim3(256), 0, current_stream(), n2,
+                     reinterpret_cast<const dbl2 *>(a), reinterpret_cast<dbl2 *>(b));
+}
+template <int U, bool NT>
+static void launch_axpy2(unsigned grid, int64_t n2, double alpha, const double *x, double beta,
+                         double *y) {
+  hipLaunchKernelGGL((k_axpy2<U, NT>), dim3(grid), dim3(256), 0, current_stream(), n2, alpha,
+                     reinterpret_cast<const dbl2 *>(x), beta, reinterpret_cast<dbl2 *>(y));
+}
+
+// The FAST grid never exceeds kFastGridMax blocks, and spx_reduce_tmp_count
+// reports kFastGridMax for FAST mode, so a partials buffer sized by it stays
+// large enough whatever tuning is set later.
+constexpr int kFastGridMax = 4096;
+static int fast_grid() {
+  spx_tuning_t t = tuning_for("reduce_fast", stream_defaults());
+  int g = t.grid_blocks > 0 ? t.grid_blocks : 2048;
+  return g < 1 ? 1 : (g > kFastGridMax ? kFastGridMax : g);
+}
+
+template <int KIND, int BS>
+static void launch_ref1(int64_t n, int64_t nb, const RedArgs &A, double *tmp, int unroll) {
+  if (unroll >= 32)
+    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 32>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n, nb, A, tmp);
+  else if (unroll >= 16)
+    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 16>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n, nb, A, tmp);
+  else
+    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 8>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n, nb, A, tmp);
+}
+
+template <int KIND>
+static int run_reduction(const char *name, int64_t n, const RedArgs &A, double *tmp, double *out,
+                         int mode, int bs) {
+  SPX_NEED_DEVICE();
+  if (n < 0) return fail(SPX_ERR_ARG, std::string(name) + ": negative n");
+  if (!tmp || !out) return fail(SPX_ERR_ARG, std::string(name) + ": null tmp/out");
+  if (mode == SPX_REDUCE_REFERENCE) {
+    if (bs != 256 && bs != 512 && bs != 1024)
+      return fail(SPX_ERR_ARG, std::string(name) + ": block_size must be 256, 512 or 1024");
+    int64_t nb = (n + bs - 1) / bs;
+    if (nb > bs) nb = bs;
+    spx_tuning_t t = tuning_for("reduce_reference", [] {
+      spx_tuning_t d{};
+      d.unroll = 16;
+      return d;
+    }());
+    const int U = t.unroll > 0 ? t.unroll : 16;
+    if (nb > 0) {
+      if (bs == 256) launch_ref1<KIND, 256>(n, nb, A, tmp, U);
+      else if (bs == 512) launch_ref1<KIND, 512>(n, nb, A, tmp, U);
+      else launch_ref1<KIND, 1024>(n, nb, A, tmp, U);
+      SPX_LAUNCHED(name);
+    }
+    // n == 0: the reference still runs stage 2 over zero partials -> 0.0
+    if (bs == 256) hipLaunchKernelGGL(k_red_ref2<256>, dim3(1), dim3(256), 0, current_stream(), nb, tmp, out);
+    else if (bs == 512) hipLaunchKernelGGL(k_red_ref2<512>, dim3(1), dim3(512), 0, current_stream(), nb, tmp, out);
+    else hipLaunchKernelGGL(k_red_ref2<1024>, dim3(1), dim3(1024), 0, current_stream(), nb, tmp, out);
+    SPX_LAUNCHED(name);
+    return SPX_OK;
+  }
+  if (mode != SPX_REDUCE_FAST) return fail(SPX_ERR_ARG, std::string(name) + ": unknown reduction mode");
+  const int g = fast_grid();
+  bool vec = aligned(A.a, 16) && (!A.b || aligned(A.b, 16)) && (!A.r || aligned(A.r, 16)) &&
+             (!A.x || aligned(A.x, 16));
+  hipLaunchKernelGGL((k_red_fast1<KIND, 4>), dim3(g), dim3(256), 0, current_stream(), n, A, tmp,
+                     vec ? 1 : 0);
+  SPX_LAUNCHED(name);
+  hipLaunchKernelGGL(k_red_fast2, dim3(1), dim3(256), 0, current_stream(), (int64_t)g, tmp, out);
+  SPX_LAUNCHED(name);
+  return SPX_OK;
+}
+
+// One stage at a time, with the caller's block count, exactly as the OKL
+// kernel pair is launched by bsNRun.cpp.
+template <int KIND>
+static int run_stage1(const char *name, int64_t nblocks, int64_t n, const RedArgs &A, double *partials,
+                      int bs) {
+  SPX_NEED_DEVICE();
+  if (n < 0 || nblocks < 0) return fail(SPX_ERR_ARG, std::string(name) + ": negative size");
+  if (bs != 256 && bs != 512 && bs != 1024)
+    return fail(SPX_ERR_ARG, std::string(name) + ": block_size must be 256, 512 or 1024");
+  if (nblocks > INT32_MAX) return fail(SPX_ERR_ARG, std::string(name) + ": too many blocks");
+  if (nblocks == 0) return SPX_OK;
+  if (!partials) return fail(SPX_ERR_ARG, std::string(name) + ": null partials");
+  spx_tuning_t t = tuning_for("reduce_reference", [] {
+    spx_tuning_t d{};
+    d.unroll = 16;
+    return d;
+  }());
+  const int U = t.unroll > 0 ? t.unroll : 16;
+  if (bs == 256) launch_ref1<KIND, 256>(n, nblocks, A, partials, U);
+  else if (bs == 512) launch_ref1<KIND, 512>(n, nblocks, A, partials, U);
+  else launch_ref1<KIND, 1024>(n, nblocks, A, partials, U);
+  SPX_LAUNCHED(name);
+  return SPX_OK;
+}
+
+extern "C" {
+
+int spx_bs0(int32_t n, const double *a, double *b) {
+  SPX_NEED_DEVICE();
+  hipLaunchKernelGGL(k_bs0, dim3(1), dim3(1), 0, current_stream(), (int)n, a, b);
+  SPX_LAUNCHED("spx_bs0");
+  return SPX_OK;
+}
+
+int spx_bs1_copy(int64_t n, const double *a, double *b) {
+  SPX_NEED_DEVICE();
+  if (n < 0) return fail(SPX_ERR_ARG, "spx_bs1_copy: negative n");
+  if (n == 0) return SPX_OK;
+  if (!a || !b) return fail(SPX_ERR_ARG, "spx_bs1_copy: null pointer");
+  spx_tuning_t t = tuning_for("bs1", stream_defaults());
+  if (aligned(a, 16) && aligned(b, 16)) {
+    const int64_t n2 = n / 2;
+    const int U = t.unroll;
+    const bool nt = t.xcd_remap == 2;  // reuse the field as the nontemporal switch
+    if (n2 > 0) {
+      unsigned grid;
+      int rc = vec_blocks(n2, (U >= 8 ? 8 : U >= 4 ? 4 : 2) * 256, &grid);
+      if (rc) return rc;
+      if (U >= 8) nt ? launch_copy2<8, true>(grid, n2, a, b) : launch_copy2<8, false>(grid, n2, a, b);
+      else if (U >= 4) nt ? launch_copy2<4, true>(grid, n2, a, b) : launch_copy2<4, false>(grid, n2, a, b);
+      else nt ? launch_copy2<2, true>(grid, n2, a, b) : launch_copy2<2, false>(grid, n2, a, b);
+      SPX_LAUNCHED("spx_bs1_copy");
+    }
+    if (n & 1) {
+      hipLaunchKernelGGL(k_copy1, dim3(1), dim3(256), 0, current_stream(), n - 1, n, a, b);
+      SPX_LAUNCHED("spx_bs1_copy tail");
+    }
+  } else {
+    int64_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_copy1, dim3((unsigned)g), dim3(256), 0, current_stream(), (int64_t)0, n, a, b);
+    SPX_LAUNCHED("spx_bs1_copy");
+  }
+  return SPX_OK;
+}
+
+int spx_bs2_axpy(int64_t n, double alpha, const double *x, double beta, double *y) {
+  SPX_NEED_DEVICE();
+  if (n < 0) return fail(SPX_ERR_ARG, "spx_bs2_axpy: negative n");
+  if (n == 0) return SPX_OK;
+  if (!x || !y) return fail(SPX_ERR_ARG, "spx_bs2_axpy: null pointer");
+  spx_tuning_t t = tuning_for("bs2", stream_defaults());
+  if (aligned(x, 16) && aligned(y, 16)) {
+    const int64_t n2 = n / 2;
+    const int U = t.unroll;
+    const bool nt = t.xcd_remap == 2;
+    if (n2 > 0) {
+      unsigned grid;
+      int rc = vec_blocks(n2, (U >= 8 ? 8 : U >= 4 ? 4 : 2) * 256, &grid);
+      if (rc) return rc;
+      if (U >= 8) nt ? launch_axpy2<8, true>(grid, n2, alpha, x, beta, y) : launch_axpy2<8, false>(grid, n2, alpha, x, beta, y);
+      else if (U >= 4) nt ? launch_axpy2<4, true>(grid, n2, alpha, x, beta, y) : launch_axpy2<4, false>(grid, n2, alpha, x, beta, y);
+      else nt ? launch_axpy2<2, true>(grid, n2, alpha, x, beta, y) : launch_axpy2<2, false>(grid, n2, alpha, x, beta, y);
+      SPX_LAUNCHED("spx_bs2_axpy");
+    }
+    if (n & 1) {
+      hipLaunchKernelGGL(k_axpy1, dim3(1), dim3(256), 0, current_stream(), n - 1, n, alpha, x, beta, y);
+      SPX_LAUNCHED("spx_bs2_axpy tail");
+    }
+  } else {
+    int64_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_axpy1, dim3((unsigned)g), dim3(256), 0, current_stream(), (int64_t)0, n,
+                       alpha, x, beta, y);
+    SPX_LAUNCHED("spx_bs2_axpy");
+  }
+  return SPX_OK;
+}
+
+int spx_bs3_1(int64_t nblocks, int64_t n, const double *a, double *partials, int bs) {
+  if (n > 0 && nblocks > 0 && !a) return fail(SPX_ERR_ARG, "spx_bs3_1: null a");
+  RedArgs A{a, nullptr, nullptr, nullptr, 0.0};
+  return run_stage1<RED_NORM>("spx_bs3_1", nblocks, n, A, partials, bs);
+}
+
+int spx_bs4_1(int64_t nblocks, int64_t n, const double *a, const double *b, double *partials, int bs) {
+  if (n > 0 && nblocks > 0 && (!a || !b)) return fail(SPX_ERR_ARG, "spx_bs4_1: null input");
+  RedArgs A{a, b, nullptr, nullptr, 0.0};
+  return run_stage1<RED_DOT>("spx_bs4_1", nblocks, n, A, partials, bs);
+}
+
+int spx_bs5_1(int64_t nblocks, int64_t n, const double *Ap, double alpha, double *r, double *partials,
+              int bs) {
+  if (n > 0 && nblocks > 0 && (!Ap || !r)) return fail(SPX_ERR_ARG, "spx_bs5_1: null input");
+  RedArgs A{Ap, nullptr, r, nullptr, alpha};
+  return run_stage1<RED_CG>("spx_bs5_1", nblocks, n, A, partials, bs);
+}
+
+int spx_reduce_2(int64_t nblocks, const double *partials, double *out, int bs) {
+  SPX_NEED_DEVICE();
+  if (nblocks < 0) return fail(SPX_ERR_ARG, "spx_reduce_2: negative nblocks");
+  if (bs != 256 && bs != 512 && bs != 1024)
+    return fail(SPX_ERR_ARG, "spx_reduce_2: block_size must be 256, 512 or 1024");
+  if (!out || (nblocks > 0 && !partials)) return fail(SPX_ERR_ARG, "spx_reduce_2: null pointer");
+  if (bs == 256) hipLaunchKernelGGL(k_red_ref2<256>, dim3(1), dim3(256), 0, current_stream(), nblocks, partials, out);
+  else if (bs == 512) hipLaunchKernelGGL(k_red_ref2<512>, dim3(1), dim3(512), 0, current_stream(), nblocks, partials, out);
+  else hipLaunchKernelGGL(k_red_ref2<1024>, dim3(1), dim3(1024), 0, current_stream(), nblocks, partials, out);
+  SPX_LAUNCHED("spx_reduce_2");
+  return SPX_OK;
+}
+
+int64_t spx_reduce_tmp_count(int64_t n, int mode, int block_size) {
+  if (mode == SPX_REDUCE_REFERENCE) {
+    if (block_size <= 0) return -1;
+    int64_t nb = (n + block_size - 1) / block_size;
+    if (nb > block_size) nb = block_size;
+    return nb > 1 ? nb : 1;
+  }
+  return kFastGridMax;
+}
+
+int spx_bs3_norm(int64_t n, const double *a, double *tmp, double *out, int mode, int bs) {
+  if (n > 0 && !a) return fail(SPX_ERR_ARG, "spx_bs3_norm: null a");
+  RedArgs A{a, nullptr, nullptr, nullptr, 0.0};
+  return run_reduction<RED_NORM>("spx_bs3_norm", n, A, tmp, out, mode, bs);
+}
+
+int spx_bs4_dot(int64_t n, const double *a, const double *b, double *tmp, double *out, int mode,
+                int bs) {
+  if (n > 0 && (!a || !b)) return fail(SPX_ERR_ARG, "spx_bs4_dot: null input");
+  RedArgs A{a, b, nullptr, nullptr, 0.0};
+  return run_reduction<RED_DOT>("spx_bs4_dot", n, A, tmp, out, mode, bs);
+}
+
+int spx_bs5_update(int64_t n, const double *Ap, double alpha, double *r, double *tmp, double *rdotr,
+                   int mode, int bs) {
+  if (n > 0 && (!Ap || !r)) return fail(SPX_ERR_ARG, "spx_bs5_update: null input");
+  RedArgs A{Ap, nullptr, r, nullptr, alpha};
+  return run_reduction<RED_CG>("spx_bs5_update", n, A, tmp, rdotr, mode, bs);
+}
+
+int spx_bs5_full(int64_t n, double alpha, const double *p, const double *Ap, double *x, double *r,
+                 double *tmp, double *rdotr, int mode, int bs) {
+  if (n > 0 && (!p || !Ap || !x || !r)) return fail(SPX_ERR_ARG, "spx_bs5_full: null input");
+  RedArgs A{p, Ap, r, x, alpha};
+  return run_reduction<RED_CGFULL>("spx_bs5_full", n, A, tmp, rdotr, mode, bs);
+}
+
+}  // extern "C"

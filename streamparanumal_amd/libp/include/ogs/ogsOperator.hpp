@@ -1,0 +1,76 @@
+// The gather operator: a CSR with one row per gathered node
+// (include/ogs/ogsOperator.hpp:36-138).  T-CSR holds every member of a group,
+// N-CSR only the unflagged (positive) ones.  Rows are cut into row blocks of at
+// most `blockNnz` column entries, the LDS tile of the gfx950 kernels.
+#pragma once
+
+#include "ogs.hpp"
+
+namespace libp {
+namespace ogs {
+
+// Reference defaults (ogsOperator.hpp:37-38) and this build's default tile.
+constexpr int blockSize = 256;
+constexpr int gatherNodesPerBlock = 512;
+constexpr int defaultBlockNnz = 2048;
+constexpr int maxBlockNnz = 4096;
+
+class ogsOperator_t {
+ public:
+  platform_t platform;
+
+  dlong Ncols = 0;
+  dlong NrowsN = 0;
+  dlong NrowsT = 0;
+  dlong nnzN = 0;
+  dlong nnzT = 0;
+
+  memory<dlong> rowStartsN, rowStartsT, colIdsN, colIdsT;
+  deviceMemory<dlong> o_rowStartsN, o_rowStartsT, o_colIdsN, o_colIdsT;
+
+  dlong blockNnz = defaultBlockNnz;  // LDS tile (max column entries per block)
+  dlong NrowBlocksN = 0, NrowBlocksT = 0;
+  memory<dlong> blockRowStartsN, blockRowStartsT;
+  deviceMemory<dlong> o_blockRowStartsN, o_blockRowStartsT;
+
+  Kind kind = Signed;
+
+  ogsOperator_t() = default;
+  // CSR arrays are taken as built (rows sorted, columns ascending in a row).
+  ogsOperator_t(platform_t &platform_, Kind kind_, const dlong NrowsN_, const dlong NrowsT_,
+                const dlong Ncols_, memory<dlong> rowStartsT_, memory<dlong> colIdsT_,
+                memory<dlong> rowStartsN_, memory<dlong> colIdsN_, int blockNnz_ = defaultBlockNnz);
+
+  void Free();
+  // Re-cut the row blocks for another LDS tile size (autotuner).
+  void SetBlockNnz(int nnz);
+
+  template <typename T> void Gather(deviceMemory<T> gv, const deviceMemory<T> v, const int k, const Op op, const Transpose trans);
+  template <typename T> void Scatter(deviceMemory<T> v, const deviceMemory<T> gv, const int k, const Transpose trans);
+  template <typename T> void GatherScatter(deviceMemory<T> v, const int k, const Op op, const Transpose trans);
+
+  // type-erased forms used by the C API (pointers are device addresses)
+  void GatherRaw(Type type, Op op, Transpose trans, int k, const void *v, void *gv);
+  void ScatterRaw(Type type, Transpose trans, int k, const void *gv, void *v);
+  void GatherScatterRaw(Type type, Op op, Transpose trans, int k, void *v);
+};
+
+// The local operator's CSR pair as built by the setup (host arrays).
+struct LocalCSR {
+  dlong NrowsT = 0, NrowsN = 0;
+  dlong nnzT = 0, nnzN = 0;  // Unsigned: N aliases T and nnzN = nnzT (ogsOperator.cpp:644-650)
+  memory<dlong> rowStartsT, colIdsT, rowStartsN, colIdsN;
+  bool gather_defined = false;
+};
+
+// Host part of ogsBase_t::Setup on one rank: grouping, unique flagging
+// (written back into ids), row numbering and both CSRs.
+LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const bool unique, const int rank);
+
+// Greedy row blocking: consecutive rows, at most `cap` entries per block.
+// Aborts when one row alone exceeds cap (ogsOperator.cpp:535-536).
+void blockRowsGreedy(const dlong Nrows, const memory<dlong> rowStarts, const int cap, dlong &Nblocks,
+                     memory<dlong> &blockStarts);
+
+}  // namespace ogs
+}  // namespace libp

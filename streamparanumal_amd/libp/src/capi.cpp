@@ -1,0 +1,258 @@
+// C ABI of the host layer (include/spx_host.h): ogs_t and mesh ids for
+// non-C++ callers.  Exceptions become SPX_ERR_* codes plus a message.
+#include <string>
+
+#include "mesh.hpp"
+#include "ogs.hpp"
+#include "spx_host.h"
+
+using namespace libp;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct host_state {
+  settings_t settings{comm_t::world()};
+  platform_t platform;
+  bool open = false;
+};
+host_state &state() {
+  static host_state s;
+  return s;
+}
+
+int err(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    return SPX_OK;
+  } catch (const libp::exception &e) {
+    return err(SPX_ERR_ARG, e.message());
+  } catch (const std::bad_alloc &) {
+    return err(SPX_ERR_ARG, "host allocation failed");
+  } catch (const std::exception &e) {
+    return err(SPX_ERR_ARG, e.what());
+  }
+}
+
+}  // namespace
+
+struct spxh_ogs_s {
+  ogs::ogs_t ogs;
+};
+
+extern "C" {
+
+const char *spxh_last_error(void) { return g_err.c_str(); }
+
+int spxh_platform_open(int device_id) {
+  return guarded([&] {
+    host_state &s = state();
+    if (s.open) {
+      LIBP_ABORT("platform already open on device " << s.platform.device.id(), s.platform.device.id() != device_id);
+      return;
+    }
+    platformAddSettings(s.settings);
+    s.settings.changeSetting("DEVICE NUMBER", std::to_string(device_id));
+    s.platform = platform_t(s.settings);
+    s.open = true;
+  });
+}
+
+int spxh_ogs_setup(int32_t n, int64_t *ids, int kind, int unique, int verbose, spxh_ogs_t *out) {
+  if (!out) return err(SPX_ERR_ARG, "spxh_ogs_setup: null out");
+  if (n > 0 && !ids) return err(SPX_ERR_ARG, "spxh_ogs_setup: null ids");
+  if (!state().open) return err(SPX_ERR_NODEVICE, "spxh_ogs_setup: call spxh_platform_open first");
+  if (kind < 0 || kind > 2) return err(SPX_ERR_ARG, "spxh_ogs_setup: kind must be 0 (Unsigned), 1 (Signed) or 2 (Halo)");
+  *out = nullptr;
+  return guarded([&] {
+    auto h = std::make_unique<spxh_ogs_s>();
+    memory<hlong> m(n);
+    if (n) std::copy(ids, ids + n, m.ptr());
+    h->ogs.Setup(n, m, comm_t::world(), (ogs::Kind)kind, ogs::Auto, unique != 0, verbose != 0, state().platform);
+    if (n) std::copy(m.ptr(), m.ptr() + n, ids);
+    *out = h.release();
+  });
+}
+
+int spxh_ogs_destroy(spxh_ogs_t h) {
+  delete h;
+  return SPX_OK;
+}
+
+int spxh_ogs_info(spxh_ogs_t h, spxh_ogs_info_t *info) {
+  if (!h || !info) return err(SPX_ERR_ARG, "spxh_ogs_info: null");
+  return guarded([&] {
+    const ogs::ogs_t &o = h->ogs;
+    auto op = o.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    info->N = o.N;
+    info->Ngather = o.Ngather;
+    info->NlocalT = o.NlocalT;
+    info->NlocalP = o.NlocalP;
+    info->nnzT = op->nnzT;
+    info->nnzN = op->nnzN;
+    info->NblocksT = op->NrowBlocksT;
+    info->NblocksN = op->NrowBlocksN;
+    info->block_nnz = op->blockNnz;
+    info->gather_defined = o.gather_defined ? 1 : 0;
+    info->kind = (int)o.kind;
+    info->unique = o.unique ? 1 : 0;
+    info->NgatherGlobal = o.NgatherGlobal;
+    info->setup_seconds = o.setupSeconds();
+  });
+}
+
+int spxh_ogs_copy_csr(spxh_ogs_t h, int which, int32_t *rowStarts, int32_t *colIds, int32_t *blockStarts) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_copy_csr: null");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    LIBP_ABORT("which must be 0 (T) or 1 (N)", which != 0 && which != 1);
+    const bool N = which == 1;
+    const dlong rows = N ? op->NrowsN : op->NrowsT;
+    const dlong nnz = N ? op->nnzN : op->nnzT;
+    const dlong nb = N ? op->NrowBlocksN : op->NrowBlocksT;
+    const memory<dlong> &rs = N ? op->rowStartsN : op->rowStartsT;
+    const memory<dlong> &ci = N ? op->colIdsN : op->colIdsT;
+    const memory<dlong> &bs = N ? op->blockRowStartsN : op->blockRowStartsT;
+    if (rowStarts) std::copy(rs.ptr(), rs.ptr() + rows + 1, rowStarts);
+    if (colIds && nnz) std::copy(ci.ptr(), ci.ptr() + nnz, colIds);
+    if (blockStarts && bs.isInitialized()) std::copy(bs.ptr(), bs.ptr() + nb + 1, blockStarts);
+  });
+}
+
+int spxh_ogs_set_block_nnz(spxh_ogs_t h, int32_t nnz) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_block_nnz: null");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    op->SetBlockNnz(nnz);
+  });
+}
+
+int spxh_ogs_global_to_local(spxh_ogs_t h, int32_t *g2l) {
+  if (!h || !g2l) return err(SPX_ERR_ARG, "spxh_ogs_global_to_local: null");
+  return guarded([&] {
+    memory<dlong> m(h->ogs.N);
+    h->ogs.SetupGlobalToLocalMapping(m);
+    std::copy(m.ptr(), m.ptr() + h->ogs.N, g2l);
+  });
+}
+
+static int check_enums(int type, int op, int trans) {
+  if (type < 0 || type > 3) return err(SPX_ERR_ARG, "type must be 0..3 (Float, Double, Int32, Int64)");
+  if (op < 0 || op > 3) return err(SPX_ERR_ARG, "op must be 0..3 (Add, Mul, Max, Min)");
+  if (trans < 0 || trans > 2) return err(SPX_ERR_ARG, "trans must be 0..2 (Sym, NoTrans, Trans)");
+  return SPX_OK;
+}
+
+int spxh_ogs_gather_scatter(spxh_ogs_t h, int type, int op, int trans, int k, void *v) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_gather_scatter: null handle");
+  if (int rc = check_enums(type, op, trans)) return rc;
+  if (k < 1) return err(SPX_ERR_ARG, "k must be >= 1");
+  return guarded([&] {
+    auto o = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !o);
+    o->GatherScatterRaw((ogs::Type)type, (ogs::Op)op, (ogs::Transpose)trans, k, v);
+  });
+}
+
+int spxh_ogs_gather(spxh_ogs_t h, int type, int op, int trans, int k, const void *v, void *gv) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_gather: null handle");
+  if (int rc = check_enums(type, op, trans)) return rc;
+  if (k < 1) return err(SPX_ERR_ARG, "k must be >= 1");
+  return guarded([&] {
+    LIBP_ABORT("Gather operation not well-defined.", !h->ogs.gather_defined);
+    auto o = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !o);
+    o->GatherRaw((ogs::Type)type, (ogs::Op)op, (ogs::Transpose)trans, k, v, gv);
+  });
+}
+
+int spxh_ogs_scatter(spxh_ogs_t h, int type, int trans, int k, const void *gv, void *v) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_scatter: null handle");
+  if (int rc = check_enums(type, 0, trans)) return rc;
+  if (k < 1) return err(SPX_ERR_ARG, "k must be >= 1");
+  return guarded([&] {
+    LIBP_ABORT("Gather operation not well-defined.", !h->ogs.gather_defined);
+    auto o = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !o);
+    o->ScatterRaw((ogs::Type)type, (ogs::Transpose)trans, k, gv, v);
+  });
+}
+
+struct spxh_csr_s {
+  ogs::LocalCSR csr;
+};
+
+int spxh_csr_build(int32_t n, int64_t *ids, int kind, int unique, spxh_csr_t *out) {
+  if (!out) return err(SPX_ERR_ARG, "spxh_csr_build: null out");
+  if (n > 0 && !ids) return err(SPX_ERR_ARG, "spxh_csr_build: null ids");
+  if (kind < 0 || kind > 2) return err(SPX_ERR_ARG, "spxh_csr_build: bad kind");
+  *out = nullptr;
+  return guarded([&] {
+    auto h = std::make_unique<spxh_csr_s>();
+    memory<hlong> m(n);
+    if (n) std::copy(ids, ids + n, m.ptr());
+    h->csr = ogs::BuildLocalCSR(n, m, (ogs::Kind)kind, unique != 0, 0);
+    if (n) std::copy(m.ptr(), m.ptr() + n, ids);
+    *out = h.release();
+  });
+}
+
+int spxh_csr_sizes(spxh_csr_t h, int32_t *sizes) {
+  if (!h || !sizes) return err(SPX_ERR_ARG, "spxh_csr_sizes: null");
+  const ogs::LocalCSR &c = h->csr;
+  sizes[0] = c.NrowsT;
+  sizes[1] = c.NrowsN;
+  sizes[2] = c.nnzT;
+  sizes[3] = c.nnzN;
+  sizes[4] = c.gather_defined ? 1 : 0;
+  return SPX_OK;
+}
+
+int spxh_csr_copy(spxh_csr_t h, int which, int32_t *rowStarts, int32_t *colIds) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_csr_copy: null");
+  if (which != 0 && which != 1) return err(SPX_ERR_ARG, "which must be 0 (T) or 1 (N)");
+  const ogs::LocalCSR &c = h->csr;
+  const dlong rows = which ? c.NrowsN : c.NrowsT;
+  const dlong nnz = which ? c.nnzN : c.nnzT;
+  const memory<dlong> &rs = which ? c.rowStartsN : c.rowStartsT;
+  const memory<dlong> &ci = which ? c.colIdsN : c.colIdsT;
+  if (rowStarts) std::copy(rs.ptr(), rs.ptr() + rows + 1, rowStarts);
+  if (colIds && nnz) std::copy(ci.ptr(), ci.ptr() + nnz, colIds);
+  return SPX_OK;
+}
+
+int spxh_csr_destroy(spxh_csr_t h) {
+  delete h;
+  return SPX_OK;
+}
+
+int spxh_block_rows(int32_t nrows, const int32_t *rowStarts, int32_t cap, int32_t *nblocks, int32_t *blockStarts) {
+  if (!rowStarts || !nblocks || !blockStarts) return err(SPX_ERR_ARG, "spxh_block_rows: null");
+  return guarded([&] {
+    memory<dlong> rs(nrows + 1);
+    std::copy(rowStarts, rowStarts + nrows + 1, rs.ptr());
+    memory<dlong> bs;
+    dlong nb = 0;
+    ogs::blockRowsGreedy(nrows, rs, cap, nb, bs);
+    *nblocks = nb;
+    std::copy(bs.ptr(), bs.ptr() + nb + 1, blockStarts);
+  });
+}
+
+int spxh_mesh_hex_ids(int nx, int ny, int nz, int p, int64_t *ids) {
+  if (!ids) return err(SPX_ERR_ARG, "spxh_mesh_hex_ids: null ids");
+  if (nx < 1 || ny < 1 || nz < 1 || p < 1 || p > 15) return err(SPX_ERR_ARG, "spxh_mesh_hex_ids: bad size");
+  return guarded([&] { mesh_t::HexBoxGlobalIds(nx, ny, nz, p, (hlong *)ids); });
+}
+
+}  // extern "C"

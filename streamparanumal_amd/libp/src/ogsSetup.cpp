@@ -1,0 +1,268 @@
+// ogs setup on one rank (libs/ogs/ogsSetup.cpp:87-290, 492-579).
+//
+// Produces the reference's operator exactly:
+//   * zero ids are ignored (:121-139);
+//   * members are grouped by |id|;
+//   * rows are numbered in order of each group's first appearance in local
+//     index order, groups holding a positive id first (:492-559);
+//   * a row lists its members in increasing local index (stable SortByRow,
+//     :561-579);
+//   * with unique=true every group keeps one positive member, picked by the
+//     reference LCG seeded with the rank (:111-113, 361-382).
+// The reference reaches this with stable sorts; here the numbering is one
+// linear pass over the ids with a dense table when the ids are dense (a
+// structured mesh's GLL numbering is), else a sort.  The member kept positive
+// under unique=true is the LCG draw of its group (groups in increasing |id|)
+// taken modulo the group size over members in increasing local index; the
+// reference takes the same draw over the order an unstable std::sort leaves
+// (ogsSetup.cpp:347), which the C++ standard does not fix.
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <numeric>
+
+#include "ogs.hpp"
+
+namespace libp {
+namespace ogs {
+
+namespace {
+
+// The reference LCG (libs/primitives/random.cpp:45-121): x0 = A*(~0 ^ seed) + C,
+// draw = abs((int)x).
+void lcgInts(int64_t n, uint64_t seed, std::vector<int32_t> &out) {
+  const uint64_t A = 6364136223846793005ULL, C = 1ULL;
+  uint64_t x = A * (UINT64_MAX ^ seed) + C;
+  out.resize(n);
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t u = (uint32_t)x;
+    if (u & 0x80000000u) u = 0u - u;
+    out[i] = (int32_t)u;
+    x = A * x + C;
+  }
+}
+
+}  // namespace
+
+LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const bool unique, const int rank) {
+  LIBP_ABORT("Invalid ogs setup requested", (kind == Unsigned && unique) || (kind == Halo && unique));
+  LIBP_ABORT("ogs::Halo setups describe exchanges between ranks; this build runs one rank per GPU "
+             "and provides no halo_t", kind == Halo);
+  LIBP_ABORT("ogs Setup: negative N", N < 0);
+  LIBP_ABORT("ogs Setup: ids hold " << ids.length() << " entries, N = " << N, (size_t)N > ids.length());
+  LocalCSR out;
+  // ---- nonzero positions and |id| range
+  hlong maxAbs = 0;
+  dlong Nids = 0;
+  for (dlong i = 0; i < N; ++i) {
+    const hlong a = std::llabs(ids[i]);
+    if (a) {
+      ++Nids;
+      if (a > maxAbs) maxAbs = a;
+    }
+  }
+  std::vector<dlong> pos;  // local index of the k-th nonzero id
+  const bool allNonzero = (Nids == N);
+  if (!allNonzero) {
+    pos.reserve(Nids);
+    for (dlong i = 0; i < N; ++i)
+      if (ids[i]) pos.push_back(i);
+  }
+  auto posOf = [&](dlong k) { return allNonzero ? k : pos[k]; };
+
+  // ---- groups in first-appearance order; rank of each group in increasing |id|
+  std::vector<dlong> grp(Nids);
+  std::vector<dlong> rankOf;  // only needed for unique
+  dlong ngroups = 0;
+  const bool dense = maxAbs < (hlong)INT32_MAX && maxAbs <= 4 * (hlong)Nids + (1 << 20);
+  if (dense) {
+    std::vector<dlong> first(maxAbs + 1, -1);
+    for (dlong k = 0; k < Nids; ++k) {
+      const hlong a = std::llabs(ids[posOf(k)]);
+      dlong &f = first[a];
+      if (f < 0) f = ngroups++;
+      grp[k] = f;
+    }
+    if (unique) {
+      rankOf.assign(ngroups, 0);
+      dlong r = 0;
+      for (hlong a = 1; a <= maxAbs; ++a)
+        if (first[a] >= 0) rankOf[first[a]] = r++;
+    }
+  } else {
+    std::vector<std::pair<hlong, dlong>> kv(Nids);
+    for (dlong k = 0; k < Nids; ++k) kv[k] = {std::llabs(ids[posOf(k)]), k};
+    std::sort(kv.begin(), kv.end());
+    // groups in increasing |id|; first member (smallest k) opens the group
+    std::vector<dlong> groupFirst;  // by |id| rank
+    std::vector<dlong> sortedGroupOf(Nids);
+    for (dlong i = 0; i < Nids; ++i) {
+      if (i == 0 || kv[i].first != kv[i - 1].first) groupFirst.push_back(kv[i].second);
+      sortedGroupOf[kv[i].second] = (dlong)groupFirst.size() - 1;
+    }
+    ngroups = (dlong)groupFirst.size();
+    std::vector<dlong> order(ngroups);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](dlong x, dlong y) { return groupFirst[x] < groupFirst[y]; });
+    std::vector<dlong> fa(ngroups);  // |id| rank -> first-appearance index
+    for (dlong j = 0; j < ngroups; ++j) fa[order[j]] = j;
+    for (dlong k = 0; k < Nids; ++k) grp[k] = fa[sortedGroupOf[k]];
+    if (unique) {
+      rankOf.assign(ngroups, 0);
+      for (dlong s = 0; s < ngroups; ++s) rankOf[fa[s]] = s;
+    }
+  }
+
+  // ---- members of each group, in increasing local index (counting sort)
+  std::vector<dlong> gStart(ngroups + 1, 0);
+  for (dlong k = 0; k < Nids; ++k) ++gStart[grp[k] + 1];
+  for (dlong g = 0; g < ngroups; ++g) gStart[g + 1] += gStart[g];
+  std::vector<dlong> members(Nids);
+  {
+    std::vector<dlong> cur(gStart.begin(), gStart.end() - 1);
+    for (dlong k = 0; k < Nids; ++k) members[cur[grp[k]]++] = posOf(k);
+  }
+
+  // ---- signs: unique flagging or the caller's flags
+  std::vector<char> hasPos(ngroups, 0);
+  bool gather_defined = true;
+  if (unique) {
+    std::vector<int32_t> rands;
+    lcgInts(ngroups, (uint64_t)rank, rands);
+    for (dlong g = 0; g < ngroups; ++g) {
+      const dlong a = gStart[g], e = gStart[g + 1];
+      const dlong m = (dlong)((uint32_t)rands[rankOf[g]] % (uint32_t)(e - a));
+      for (dlong i = a; i < e; ++i) {
+        const hlong b = std::llabs(ids[members[i]]);
+        ids[members[i]] = (i - a == m) ? b : -b;
+      }
+      hasPos[g] = 1;
+    }
+  } else {
+    for (dlong g = 0; g < ngroups; ++g) {
+      int npos = 0;
+      for (dlong i = gStart[g]; i < gStart[g + 1]; ++i) npos += (ids[members[i]] > 0);
+      hasPos[g] = npos > 0;
+      if (npos != 1) gather_defined = false;
+    }
+  }
+
+  // ---- rows: positive groups first, each class in first-appearance order
+  std::vector<dlong> rowOf(ngroups);
+  dlong NP = 0;
+  for (dlong g = 0; g < ngroups; ++g)
+    if (hasPos[g]) rowOf[g] = NP++;
+  {
+    dlong r = NP;
+    for (dlong g = 0; g < ngroups; ++g)
+      if (!hasPos[g]) rowOf[g] = r++;
+  }
+
+  // ---- T-CSR (every member) and N-CSR (positive members)
+  memory<dlong> rowStartsT(ngroups + 1), colIdsT(Nids);
+  rowStartsT[0] = 0;
+  {
+    std::vector<dlong> sizeOfRow(ngroups);
+    for (dlong g = 0; g < ngroups; ++g) sizeOfRow[rowOf[g]] = gStart[g + 1] - gStart[g];
+    for (dlong r = 0; r < ngroups; ++r) rowStartsT[r + 1] = rowStartsT[r] + sizeOfRow[r];
+    for (dlong g = 0; g < ngroups; ++g)
+      std::copy(members.begin() + gStart[g], members.begin() + gStart[g + 1], colIdsT.ptr() + rowStartsT[rowOf[g]]);
+  }
+  memory<dlong> rowStartsN, colIdsN;
+  dlong nnzN = 0;
+  if (kind == Signed) {
+    rowStartsN.malloc(NP + 1);
+    rowStartsN[0] = 0;
+    for (dlong r = 0; r < NP; ++r) {
+      dlong c = 0;
+      for (dlong i = rowStartsT[r]; i < rowStartsT[r + 1]; ++i) c += (ids[colIdsT[i]] > 0);
+      rowStartsN[r + 1] = rowStartsN[r] + c;
+    }
+    nnzN = rowStartsN[NP];
+    colIdsN.malloc(nnzN);
+    dlong c = 0;
+    for (dlong r = 0; r < NP; ++r)
+      for (dlong i = rowStartsT[r]; i < rowStartsT[r + 1]; ++i)
+        if (ids[colIdsT[i]] > 0) colIdsN[c++] = colIdsT[i];
+  } else {
+    rowStartsN = rowStartsT;
+    colIdsN = colIdsT;
+    nnzN = Nids;
+  }
+
+  out.NrowsT = ngroups;
+  out.nnzT = Nids;
+  out.nnzN = nnzN;
+  out.NrowsN = NP;
+  out.rowStartsT = rowStartsT;
+  out.colIdsT = colIdsT;
+  out.rowStartsN = rowStartsN;
+  out.colIdsN = colIdsN;
+  out.gather_defined = gather_defined;
+  return out;
+}
+
+void ogsBase_t::Setup(const dlong _N, memory<hlong> ids, comm_t _comm, const Kind _kind, const Method method,
+                      const bool _unique, const bool verbose, platform_t &_platform) {
+  (void)method;  // one rank: no exchange to choose
+  Free();
+  const auto t0 = std::chrono::steady_clock::now();
+
+  platform = _platform;
+  comm = _comm;
+  N = _N;
+  kind = _kind;
+  unique = _unique;
+
+  LocalCSR csr = BuildLocalCSR(N, ids, kind, unique, comm.rank());
+  gather_defined = csr.gather_defined;
+
+  NlocalT = csr.NrowsT;
+  NlocalP = csr.NrowsN;
+  NhaloT = 0;
+  NhaloP = 0;
+  Ngather = NlocalP + NhaloP;
+  NgatherGlobal = Ngather;
+  comm.Allreduce(NgatherGlobal);
+
+  gatherLocal = std::make_shared<ogsOperator_t>(platform, kind == Unsigned ? Unsigned : Signed, csr.NrowsN,
+                                                csr.NrowsT, N, csr.rowStartsT, csr.colIdsT, csr.rowStartsN,
+                                                csr.colIdsN);
+
+  setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (verbose && comm.rank() == 0)
+    std::cout << "ogs Setup: " << NlocalT << " gather nodes, " << gatherLocal->nnzT << " entries; setup "
+              << setupSeconds_ << " seconds." << std::endl;
+}
+
+void ogsBase_t::Free() {
+  comm.Free();
+  gatherLocal = nullptr;
+  N = 0;
+  NlocalT = 0;
+  NhaloT = 0;
+  NlocalP = 0;
+  NhaloP = 0;
+  Ngather = 0;
+  NgatherGlobal = 0;
+}
+
+void ogsBase_t::AssertGatherDefined() { LIBP_ABORT("Gather operation not well-defined.", !gather_defined); }
+
+void ogs_t::Setup(const dlong _N, memory<hlong> ids, comm_t _comm, const Kind _kind, const Method method,
+                  const bool _unique, const bool verbose, platform_t &_platform) {
+  ogsBase_t::Setup(_N, ids, _comm, _kind, method, _unique, verbose, _platform);
+}
+
+// ogsSetup.cpp:833-849: GlobalToLocal[l] = gathered row of local node l, or -1.
+void ogs_t::SetupGlobalToLocalMapping(memory<dlong> GlobalToLocal) {
+  LIBP_ABORT("ogs handle is not set up.", NgatherGlobal == 0);
+  LIBP_ABORT("GlobalToLocal must hold N entries", GlobalToLocal.length() < (size_t)N);
+  for (dlong i = 0; i < N; ++i) GlobalToLocal[i] = -1;
+  const ogsOperator_t &op = *gatherLocal;
+  for (dlong r = 0; r < op.NrowsT; ++r)
+    for (dlong i = op.rowStartsT[r]; i < op.rowStartsT[r + 1]; ++i) GlobalToLocal[op.colIdsT[i]] = r;
+}
+
+}  // namespace ogs
+}  // namespace libp

@@ -1,0 +1,128 @@
+"""Python mirror of ogs_t (include/ogs.hpp:212-344) over libspxhost.
+
+    ogs = Ogs(ids, kind=SIGNED, unique=True)       # ogs_t::Setup
+    ogs.gather_scatter(q, k=1, op=ADD, trans=SYM)  # ogs_t::GatherScatter
+    ogs.gather(gq, q, ...); ogs.scatter(q, gq, ...)
+
+Device arrays are streamparanumal_amd.device.DeviceArray; work is enqueued on
+the current stream.  The operator lives on the device; host copies of its CSR
+are available for inspection.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import checkh, libspxhost
+from .device import ogs_type
+
+_platform_open = False
+
+
+def platform_open(device_id=0):
+    """platform_t in HIP mode on `device_id` (opens the device once)."""
+    global _platform_open
+    if not _platform_open:
+        checkh(libspxhost().spxh_platform_open(device_id), "spxh_platform_open")
+        _platform_open = True
+
+
+def hex_ids(nx, ny, nz, p):
+    """Lexicographic GLL-grid ids of a hex box (meshSetupBoxHex3D.cpp:87-130)."""
+    n = nx * ny * nz * (p + 1) ** 3
+    ids = np.empty(n, dtype=np.int64)
+    checkh(libspxhost().spxh_mesh_hex_ids(nx, ny, nz, p, ids.ctypes.data), "spxh_mesh_hex_ids")
+    return ids
+
+
+def build_csr(ids, kind=_lib.SIGNED, unique=True):
+    """Host half of the setup, no device: returns (ids_after, dict of CSR arrays)."""
+    ids = np.ascontiguousarray(ids, dtype=np.int64).copy()
+    h = C.c_void_p()
+    checkh(libspxhost().spxh_csr_build(ids.size, ids.ctypes.data, kind, 1 if unique else 0, C.byref(h)),
+           "spxh_csr_build")
+    try:
+        sz = np.zeros(5, dtype=np.int32)
+        checkh(libspxhost().spxh_csr_sizes(h, sz.ctypes.data), "spxh_csr_sizes")
+        NrowsT, NrowsN, nnzT, nnzN, gd = (int(x) for x in sz)
+        out = {"NrowsT": NrowsT, "NrowsN": NrowsN, "gather_defined": bool(gd)}
+        for which, tag, rows, nnz in ((0, "T", NrowsT, nnzT), (1, "N", NrowsN, nnzN)):
+            rs = np.zeros(rows + 1, dtype=np.int32)
+            ci = np.zeros(max(nnz, 1), dtype=np.int32)
+            checkh(libspxhost().spxh_csr_copy(h, which, rs.ctypes.data, ci.ctypes.data), "spxh_csr_copy")
+            out["rowStarts" + tag] = rs
+            out["colIds" + tag] = ci[:nnz]
+        return ids, out
+    finally:
+        libspxhost().spxh_csr_destroy(h)
+
+
+def block_rows(row_starts, cap):
+    """Greedy row blocks of at most `cap` entries (the product's partition)."""
+    rs = np.ascontiguousarray(row_starts, dtype=np.int32)
+    nrows = rs.size - 1
+    bs = np.zeros(nrows + 2, dtype=np.int32)
+    nb = C.c_int32()
+    checkh(libspxhost().spxh_block_rows(nrows, rs.ctypes.data, cap, C.byref(nb), bs.ctypes.data),
+           "spxh_block_rows")
+    return bs[: nb.value + 1]
+
+
+class Ogs:
+    """ogs_t set up on one rank from an id list (ogsSetup.cpp:62-71)."""
+
+    def __init__(self, ids, kind=_lib.SIGNED, unique=True, verbose=False, device_id=0):
+        platform_open(device_id)
+        ids = np.ascontiguousarray(ids, dtype=np.int64).copy()
+        self.h = C.c_void_p()
+        checkh(libspxhost().spxh_ogs_setup(ids.size, ids.ctypes.data, kind, 1 if unique else 0,
+                                           1 if verbose else 0, C.byref(self.h)), "spxh_ogs_setup")
+        self.ids = ids  # sign-flagged when unique
+        self.refresh()
+
+    def refresh(self):
+        info = _lib.OgsInfo()
+        checkh(libspxhost().spxh_ogs_info(self.h, C.byref(info)), "spxh_ogs_info")
+        self.info = {f: getattr(info, f) for f, _ in _lib.OgsInfo._fields_}
+        for k in ("N", "Ngather", "NlocalT", "NlocalP", "NgatherGlobal", "gather_defined"):
+            setattr(self, k, self.info[k])
+
+    def csr(self, which="T"):
+        w = 0 if which == "T" else 1
+        rows = self.info["NlocalT"] if w == 0 else self.info["NlocalP"]
+        nnz = self.info["nnzT"] if w == 0 else self.info["nnzN"]
+        nb = self.info["NblocksT"] if w == 0 else self.info["NblocksN"]
+        rs = np.zeros(rows + 1, dtype=np.int32)
+        ci = np.zeros(max(nnz, 1), dtype=np.int32)
+        bs = np.zeros(nb + 1, dtype=np.int32)
+        checkh(libspxhost().spxh_ogs_copy_csr(self.h, w, rs.ctypes.data, ci.ctypes.data, bs.ctypes.data),
+               "spxh_ogs_copy_csr")
+        return rs, ci[:nnz], bs
+
+    def set_block_nnz(self, cap):
+        checkh(libspxhost().spxh_ogs_set_block_nnz(self.h, cap), "spxh_ogs_set_block_nnz")
+        self.refresh()
+
+    def global_to_local(self):
+        g = np.empty(self.N, dtype=np.int32)
+        checkh(libspxhost().spxh_ogs_global_to_local(self.h, g.ctypes.data), "spxh_ogs_global_to_local")
+        return g
+
+    def gather_scatter(self, q, k=1, op=_lib.ADD, trans=_lib.SYM):
+        checkh(libspxhost().spxh_ogs_gather_scatter(self.h, ogs_type(q.dtype), op, trans, k, q.ptr),
+               "spxh_ogs_gather_scatter")
+
+    def gather(self, gq, q, k=1, op=_lib.ADD, trans=_lib.TRANS):
+        checkh(libspxhost().spxh_ogs_gather(self.h, ogs_type(q.dtype), op, trans, k, q.ptr, gq.ptr),
+               "spxh_ogs_gather")
+
+    def scatter(self, q, gq, k=1, trans=_lib.NOTRANS):
+        checkh(libspxhost().spxh_ogs_scatter(self.h, ogs_type(q.dtype), trans, k, gq.ptr, q.ptr),
+               "spxh_ogs_scatter")
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                libspxhost().spxh_ogs_destroy(self.h)
+        except Exception:
+            pass

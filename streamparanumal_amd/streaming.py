@@ -1,0 +1,62 @@
+"""BS0-BS5 streaming operations on device arrays (BS/BSn/okl/bsN.okl)."""
+import numpy as np
+
+from . import _lib
+from ._lib import check, libspx
+from .device import DeviceArray
+
+
+def bs0(n, a, b):
+    """bs0.okl:27-39 -- one empty 1x1 launch."""
+    check(libspx().spx_bs0(n, a.ptr, b.ptr), "spx_bs0")
+
+
+def copy(a, b, n=None):
+    """BS1: b = a (bs1.okl:27-34)."""
+    n = a.n if n is None else n
+    check(libspx().spx_bs1_copy(n, a.ptr, b.ptr), "spx_bs1_copy")
+
+
+def axpy(alpha, x, beta, y, n=None):
+    """BS2: y = alpha*x + beta*y, two products and one add (bs2.okl:27-36)."""
+    n = x.n if n is None else n
+    check(libspx().spx_bs2_axpy(n, alpha, x.ptr, beta, y.ptr), "spx_bs2_axpy")
+
+
+class Reducer:
+    """Scratch for the BS3-BS5 reductions: per-block partials and the result."""
+
+    def __init__(self, n, mode=_lib.REDUCE_REFERENCE, block_size=256):
+        self.mode, self.block_size = mode, block_size
+        cnt = libspx().spx_reduce_tmp_count(n, mode, block_size)
+        if cnt < 0:
+            raise ValueError("bad reduction parameters")
+        self.tmp = DeviceArray(max(cnt, 1))
+        self.out = DeviceArray(1)
+
+    def value(self):
+        return float(self.out.to_numpy()[0])
+
+    def norm(self, a, n=None):
+        """BS3: sum a^2 (bs3.okl:27-102)."""
+        n = a.n if n is None else n
+        check(libspx().spx_bs3_norm(n, a.ptr, self.tmp.ptr, self.out.ptr, self.mode, self.block_size),
+              "spx_bs3_norm")
+
+    def dot(self, a, b, n=None):
+        """BS4: sum a*b (bs4.okl:27-103)."""
+        n = a.n if n is None else n
+        check(libspx().spx_bs4_dot(n, a.ptr, b.ptr, self.tmp.ptr, self.out.ptr, self.mode, self.block_size),
+              "spx_bs4_dot")
+
+    def cg_update(self, Ap, alpha, r, n=None):
+        """BS5 kernel form: r += alpha*Ap; rdotr = sum r^2 (bs5.okl:27-106)."""
+        n = Ap.n if n is None else n
+        check(libspx().spx_bs5_update(n, Ap.ptr, alpha, r.ptr, self.tmp.ptr, self.out.ptr, self.mode,
+                                      self.block_size), "spx_bs5_update")
+
+    def cg_full(self, alpha, p, Ap, x, r, n=None):
+        """CG update of README.md:49-53: x += alpha p; r -= alpha Ap; rdotr = sum r^2."""
+        n = p.n if n is None else n
+        check(libspx().spx_bs5_full(n, alpha, p.ptr, Ap.ptr, x.ptr, r.ptr, self.tmp.ptr, self.out.ptr,
+                                    self.mode, self.block_size), "spx_bs5_full")

@@ -1,0 +1,222 @@
+"""GPU parity of the ogs gather / scatter / gather+scatter (BS6-BS8) against the oracle.
+
+Bit-exact for every type, op, transpose and field count: each row is folded in
+ascending local index on both sides.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import streamparanumal_amd as spx
+from streamparanumal_amd import device, ogs
+from streamparanumal_amd._lib import SpxError, check, libspx
+from streamparanumal_amd.device import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+MESHES = [(1, 1, 1, 1), (2, 1, 1, 1), (2, 2, 2, 2), (3, 2, 1, 3), (5, 3, 2, 4), (4, 4, 4, 15), (8, 8, 8, 7)]
+DTYPES = [np.float64, np.float32, np.int32, np.int64]
+OPS = [spx.ADD, spx.MUL, spx.MAX, spx.MIN]
+TRANS = [spx.SYM, spx.NOTRANS, spx.TRANS]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype.kind == "f":
+        return a.dtype == b.dtype and np.array_equal(a.view(f"u{a.itemsize}"), b.view(f"u{b.itemsize}"))
+    return np.array_equal(a, b)
+
+
+def values(n, dtype, seed, k=1):
+    from oracle import pyoracle
+    u = pyoracle.uniform(n * k, seed)
+    if np.dtype(dtype).kind == "f":
+        return u.astype(dtype)
+    return np.floor(u * 1000).astype(dtype)
+
+
+@pytest.fixture(scope="module")
+def setups(dev, oracle):
+    out = {}
+    for m in MESHES:
+        ids = ogs.hex_ids(*m)
+        out[m] = (ogs.Ogs(ids), oracle.Ogs(ids), ids)
+    return out
+
+
+@pytest.mark.parametrize("mesh", MESHES)
+def test_setup_matches_oracle(setups, mesh):
+    say("setup", mesh)
+    g, o, ids = setups[mesh]
+    assert g.NlocalT == o.NrowsT and g.NlocalP == o.NrowsN
+    rsT, ciT, _ = g.csr("T")
+    rsN, ciN, _ = g.csr("N")
+    assert same(rsT, o.rowStartsT) and same(ciT, o.colIdsT)
+    assert same(rsN, o.rowStartsN) and same(ciN, o.colIdsN)
+    assert same(g.ids, o.ids)
+
+
+@pytest.mark.parametrize("mesh", MESHES)
+def test_bs8_gather_scatter_double(setups, oracle, mesh):
+    say("bs8 gs double", mesh)
+    g, o, _ = setups[mesh]
+    q = oracle.uniform(g.N, 0)
+    dq = DeviceArray.from_numpy(q)
+    g.gather_scatter(dq)
+    device.check_kernel_errors()
+    assert same(dq.to_numpy(), o.gather_scatter(q))
+
+
+@pytest.mark.parametrize("mesh", MESHES)
+def test_bs6_gather_bs7_scatter_double(setups, oracle, mesh):
+    say("bs6/bs7 double", mesh)
+    g, o, _ = setups[mesh]
+    q = oracle.uniform(g.N, 0)
+    dq, dgq = DeviceArray.from_numpy(q), DeviceArray(g.NlocalT)
+    g.gather(dgq, dq, 1, spx.ADD, spx.TRANS)
+    gq = dgq.to_numpy()
+    assert same(gq, o.gather(q, 1, spx.ADD, spx.TRANS))
+    out = DeviceArray(g.N)
+    out.fill_zero()
+    g.scatter(out, dgq, 1, spx.NOTRANS)
+    device.check_kernel_errors()
+    assert same(out.to_numpy(), o.scatter(gq, np.zeros(g.N), 1, spx.NOTRANS))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("trans", TRANS)
+@pytest.mark.parametrize("k", [1, 2, 3, 5])
+def test_all_types_ops_trans_fields(setups, dtype, op, trans, k):
+    mesh = (3, 2, 1, 3)
+    say("gs/g/s", mesh, np.dtype(dtype).name, op, trans, k)
+    g, o, _ = setups[mesh]
+    q = values(g.N, dtype, 5, k)
+    if op == spx.MUL:  # keep products finite and exact-ish: values near 1
+        q = (np.ones_like(q) + (q % 3 if np.dtype(dtype).kind == "i" else q * 0.01)).astype(dtype)
+    dq = DeviceArray.from_numpy(q)
+    g.gather_scatter(dq, k, op, trans)
+    assert same(dq.to_numpy(), o.gather_scatter(q, k, op, trans))
+    rows = g.NlocalP if trans == spx.NOTRANS else g.NlocalT
+    dq = DeviceArray.from_numpy(q)
+    dgq = DeviceArray(rows * k, dtype)
+    g.gather(dgq, dq, k, op, trans)
+    gq = dgq.to_numpy()
+    assert same(gq, o.gather(q, k, op, trans))
+    srows = g.NlocalP if trans == spx.TRANS else g.NlocalT
+    gsrc = values(srows, dtype, 6, k)
+    base = values(g.N, dtype, 7, k)
+    dv, dg = DeviceArray.from_numpy(base), DeviceArray.from_numpy(gsrc)
+    g.scatter(dv, dg, k, trans)
+    device.check_kernel_errors()
+    assert same(dv.to_numpy(), o.scatter(gsrc, base, k, trans))
+
+
+@pytest.mark.parametrize("cap", [64, 512, 1024, 4096])
+def test_result_independent_of_row_blocks(setups, oracle, cap):
+    mesh = (5, 3, 2, 4)
+    say("block cap", mesh, cap)
+    g, o, _ = setups[mesh]
+    q = oracle.uniform(g.N, 11)
+    want = o.gather_scatter(q)
+    g.set_block_nnz(cap)
+    try:
+        dq = DeviceArray.from_numpy(q)
+        g.gather_scatter(dq)
+        device.check_kernel_errors()
+        assert same(dq.to_numpy(), want)
+    finally:
+        g.set_block_nnz(2048)
+
+
+def test_reference_bisection_blocks(setups, oracle):
+    """The kernel run on the reference's own row blocks (<=512 entries, ogsOperator.cpp:522-594)."""
+    mesh = (8, 8, 8, 7)
+    say("reference blocks", mesh)
+    g, o, _ = setups[mesh]
+    q = oracle.uniform(g.N, 0)
+    want = oracle.gs_okl(o.blockStartsT, o.rowStartsT, o.colIdsT, o.rowStartsT, o.colIdsT, q)
+    assert same(want, o.gather_scatter(q))
+    bs = DeviceArray.from_numpy(o.blockStartsT.astype(np.int32))
+    rs = DeviceArray.from_numpy(o.rowStartsT.astype(np.int32))
+    ci = DeviceArray.from_numpy(o.colIdsT.astype(np.int32))
+    dq = DeviceArray.from_numpy(q)
+    check(libspx().spx_ogs_gather_scatter(spx.DOUBLE, spx.ADD, 1, o.blockStartsT.size - 1, bs.ptr, o.NrowsT,
+                                          rs.ptr, ci.ptr, o.NrowsT, rs.ptr, ci.ptr, 512, dq.ptr), "gs")
+    device.check_kernel_errors()
+    assert same(dq.to_numpy(), want)
+
+
+def test_order_probe_corner_row(dev, oracle):
+    """{1e16, 1, -1e16} on an 8-member row: only ascending local order gives 0."""
+    ids = ogs.hex_ids(2, 2, 2, 1)
+    g, o = ogs.Ogs(ids), oracle.Ogs(ids)
+    rs, ci, _ = g.csr("T")
+    sizes = np.diff(rs)
+    r = int(np.argmax(sizes == 8))
+    members = ci[rs[r]:rs[r + 1]]
+    q = np.zeros(g.N)
+    q[members[0]], q[members[1]], q[members[2]] = 1e16, 1.0, -1e16
+    dq = DeviceArray.from_numpy(q)
+    g.gather_scatter(dq)
+    out = dq.to_numpy()
+    assert np.all(out[members] == 0.0)
+    assert same(out, o.gather_scatter(q))
+
+
+def test_oversized_block_is_reported(setups, oracle):
+    """A block larger than max_block_nnz is refused by the kernel's guard, not overrun."""
+    mesh = (2, 2, 2, 2)
+    g, o, _ = setups[mesh]
+    say("oversized block", mesh)
+    bs = DeviceArray.from_numpy(np.array([0, o.NrowsT], dtype=np.int32))
+    rs = DeviceArray.from_numpy(o.rowStartsT.astype(np.int32))
+    ci = DeviceArray.from_numpy(o.colIdsT.astype(np.int32))
+    q = oracle.uniform(g.N, 1)
+    dq = DeviceArray.from_numpy(q)
+    check(libspx().spx_ogs_gather_scatter(spx.DOUBLE, spx.ADD, 1, 1, bs.ptr, o.NrowsT, rs.ptr, ci.ptr,
+                                          o.NrowsT, rs.ptr, ci.ptr, 64, dq.ptr), "gs")
+    with pytest.raises(SpxError):
+        device.check_kernel_errors()
+    assert same(dq.to_numpy(), q)  # untouched
+
+
+def test_gather_not_defined_raises(dev):
+    ids = np.array([1, 1, 2, 2, 3], dtype=np.int64)  # no unique flagging: two positives per group
+    g = ogs.Ogs(ids, kind=spx.SIGNED, unique=False)
+    assert not g.gather_defined
+    q, gq = DeviceArray.from_numpy(np.ones(5)), DeviceArray(3)
+    with pytest.raises(SpxError):
+        g.gather(gq, q)
+
+
+def test_full_size_bs8_64cube_p7(dev, oracle):
+    """64^3 elements at p=7 (134M local nodes): CSR invariants, then BS8 bit-exact vs the oracle's
+    fold over the same CSR, seed 1."""
+    nx, p = 64, 7
+    say("full size bs8", nx, p)
+    ids = ogs.hex_ids(nx, nx, nx, p)
+    g = ogs.Ogs(ids)
+    rs, ci, _ = g.csr("T")
+    assert g.NlocalT == (nx * p + 1) ** 3 == 90518849
+    # invariants fixing the reference's operator: every local node in one row, rows in order of first
+    # appearance, members ascending, members share an id
+    assert ci.size == ids.size and np.array_equal(np.sort(ci), np.arange(ids.size, dtype=np.int32))
+    first = ci[rs[:-1]]
+    assert np.all(np.diff(first) > 0)
+    row_of = np.repeat(np.arange(g.NlocalT), np.diff(rs))
+    assert np.all(np.diff(ci)[np.diff(row_of) == 0] > 0)
+    lex = np.abs(g.ids[ci])
+    assert np.all(lex[1:][np.diff(row_of) == 0] == lex[:-1][np.diff(row_of) == 0])
+    del row_of, lex
+    q = DeviceArray(ids.size)
+    device.fill_uniform(q, 1, 0)
+    g.gather_scatter(q)
+    device.check_kernel_errors()
+    want = oracle.csr_gs(rs, ci, oracle.uniform(ids.size, 1))
+    assert same(q.to_numpy(), want)

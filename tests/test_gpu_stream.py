@@ -1,0 +1,206 @@
+"""GPU parity of BS0-BS5 against the CPU oracle (bit-exact; FAST reductions to 1e-14).
+
+Each test prints its case before the GPU work so a failing run names it.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import streamparanumal_amd as spx
+from streamparanumal_amd import device, streaming
+from streamparanumal_amd._lib import check, libspx
+from streamparanumal_amd.device import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [0, 1, 2, 3, 255, 256, 257, 1000, 65536 + 7, 262144, 1 << 20 | 3]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def test_fill_uniform_matches_oracle(dev, oracle):
+    for n, seed, skip in [(1, 1, 0), (1000, 1, 0), (4099, 0, 7), (100000, 3, 123456789)]:
+        say("fill", n, seed, skip)
+        d = DeviceArray(n)
+        device.fill_uniform(d, seed, skip)
+        assert np.array_equal(bits(d.to_numpy()), bits(oracle.uniform(n, seed, skip)))
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_copy(dev, oracle, n):
+    say("bs1 copy", n)
+    a = oracle.uniform(n, 1)
+    da, db = DeviceArray.from_numpy(a), DeviceArray(n)
+    streaming.copy(da, db)
+    assert np.array_equal(bits(db.to_numpy()), bits(oracle.bs1_copy(a)))
+
+
+def test_copy_unaligned(dev, oracle):
+    n = 10001
+    say("bs1 copy unaligned", n)
+    a = oracle.uniform(n + 1, 1)
+    da, db = DeviceArray.from_numpy(a), DeviceArray(n + 1)
+    db.fill_zero()
+    check(libspx().spx_bs1_copy(n, da.offset(1), db.offset(0)), "copy")
+    out = db.to_numpy()
+    assert np.array_equal(bits(out[:n]), bits(a[1:]))
+    assert out[n] == 0.0
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (0.37, -1.21)])
+def test_axpy(dev, oracle, n, alpha, beta):
+    say("bs2 axpy", n, alpha, beta)
+    x, y = oracle.uniform(n, 1), oracle.uniform(n, 1, n)
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y)
+    streaming.axpy(alpha, dx, beta, dy)
+    assert np.array_equal(bits(dy.to_numpy()), bits(oracle.bs2_axpy(alpha, x, beta, y)))
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("bs", [256, 512, 1024])
+def test_reductions_reference_order(dev, oracle, n, bs):
+    say("bs3/4/5 reference", n, bs)
+    a, b = oracle.uniform(n, 1), oracle.uniform(n, 1, n)
+    da, db = DeviceArray.from_numpy(a), DeviceArray.from_numpy(b)
+    red = streaming.Reducer(n, spx.REDUCE_REFERENCE, bs)
+    red.norm(da)
+    assert bits([red.value()])[0] == bits([oracle.bs3_norm(a, bs)])[0]
+    red.dot(da, db)
+    assert bits([red.value()])[0] == bits([oracle.bs4_dot(a, b, bs)])[0]
+    alpha = 0.37
+    ref, r_ref = oracle.bs5_update(a, alpha, b, bs)
+    red.cg_update(da, alpha, db)
+    assert bits([red.value()])[0] == bits([ref])[0]
+    assert np.array_equal(bits(db.to_numpy()), bits(r_ref))
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 262144, 1 << 20 | 5])
+def test_cg_full_form(dev, oracle, n):
+    say("bs5 full", n)
+    p, Ap, x, r = (oracle.uniform(n, 3, j * n) for j in range(4))
+    dp, dAp, dx, dr = (DeviceArray.from_numpy(v) for v in (p, Ap, x, r))
+    red = streaming.Reducer(n, spx.REDUCE_REFERENCE, 256)
+    red.cg_full(0.37, dp, dAp, dx, dr)
+    v, x_ref, r_ref = oracle.bs5_full(0.37, p, Ap, x, r)
+    assert bits([red.value()])[0] == bits([v])[0]
+    assert np.array_equal(bits(dx.to_numpy()), bits(x_ref))
+    assert np.array_equal(bits(dr.to_numpy()), bits(r_ref))
+
+
+def test_stage_kernels_match_reference(dev, oracle):
+    """bs3_1/bs4_1/bs5_1 + bsN_2 launched as bsNRun.cpp does (Nblock = min(ceil(N/bs), bs))."""
+    n, bs = 300001, 256
+    say("stage kernels", n, bs)
+    a, b = oracle.uniform(n, 2), oracle.uniform(n, 2, n)
+    da, db = DeviceArray.from_numpy(a), DeviceArray.from_numpy(b)
+    nb = min((n + bs - 1) // bs, bs)
+    tmp, out = DeviceArray(bs), DeviceArray(1)
+    check(libspx().spx_bs3_1(nb, n, da.ptr, tmp.ptr, bs), "bs3_1")
+    check(libspx().spx_reduce_2(nb, tmp.ptr, out.ptr, bs), "reduce_2")
+    assert bits(out.to_numpy())[0] == bits([oracle.bs3_norm(a, bs)])[0]
+    check(libspx().spx_bs4_1(nb, n, da.ptr, db.ptr, tmp.ptr, bs), "bs4_1")
+    check(libspx().spx_reduce_2(nb, tmp.ptr, out.ptr, bs), "reduce_2")
+    assert bits(out.to_numpy())[0] == bits([oracle.bs4_dot(a, b, bs)])[0]
+
+
+@pytest.mark.parametrize("n", [1, 1000, 262144, 1 << 22 | 1])
+def test_fast_reductions_tolerance_and_determinism(dev, oracle, n):
+    say("fast reductions", n)
+    a, b = oracle.uniform(n, 1), oracle.uniform(n, 1, n)
+    da, db = DeviceArray.from_numpy(a), DeviceArray.from_numpy(b)
+    red = streaming.Reducer(n, spx.REDUCE_FAST)
+    tol = 1e-14
+    vals = []
+    for _ in range(3):
+        red.norm(da)
+        vals.append(red.value())
+    assert len({v.hex() for v in vals}) == 1
+    ref = oracle.bs3_norm(a)
+    assert abs(vals[0] - ref) <= tol * abs(ref)
+    red.dot(da, db)
+    ref = oracle.bs4_dot(a, b)
+    assert abs(red.value() - ref) <= tol * abs(ref)
+    ref, r_ref = oracle.bs5_update(a, 0.37, b)
+    red.cg_update(da, 0.37, db)
+    assert abs(red.value() - ref) <= tol * abs(ref)
+    assert np.array_equal(bits(db.to_numpy()), bits(r_ref))
+
+
+def test_full_size_stream_suite(dev, oracle):
+    """N = 2^27 doubles (1 GiB per vector), seed 1, alpha=0.37, beta=-1.21: bit-exact."""
+    n = 1 << 27
+    say("full size BS1-BS5", n)
+    a = oracle.uniform(n, 1)
+    b = oracle.uniform(n, 1, n)
+    da, db = DeviceArray(n), DeviceArray(n)
+    device.fill_uniform(da, 1, 0)
+    device.fill_uniform(db, 1, n)
+    assert np.array_equal(bits(da.to_numpy()), bits(a))
+    red = streaming.Reducer(n, spx.REDUCE_REFERENCE, 256)
+    red.norm(da)
+    assert bits([red.value()])[0] == bits([oracle.bs3_norm(a)])[0]
+    red.dot(da, db)
+    assert bits([red.value()])[0] == bits([oracle.bs4_dot(a, b)])[0]
+    streaming.axpy(0.37, da, -1.21, db)
+    y = oracle.bs2_axpy(0.37, a, -1.21, b)
+    assert np.array_equal(bits(db.to_numpy()), bits(y))
+    del y
+    streaming.copy(da, db)
+    assert np.array_equal(bits(db.to_numpy()), bits(a))
+    ref, r_ref = oracle.bs5_update(a, 0.37, b)
+    device.fill_uniform(db, 1, n)
+    red.cg_update(da, 0.37, db)
+    assert bits([red.value()])[0] == bits([ref])[0]
+    assert np.array_equal(bits(db.to_numpy()), bits(r_ref))
+
+
+def test_bs0_plain_and_graph(dev):
+    say("bs0")
+    a, b = DeviceArray(1), DeviceArray(1)
+    for _ in range(10):
+        streaming.bs0(1, a, b)
+    g = device.Graph.capture(lambda: [streaming.bs0(1, a, b) for _ in range(100)])
+    for _ in range(3):
+        g.launch()
+    device.finish()
+    # n == 0 makes the single work-item store b[0] = 0
+    b.copy_from(np.array([5.0]))
+    streaming.bs0(0, a, b)
+    device.finish()
+    assert b.to_numpy()[0] == 0.0
+
+
+def test_conversions(dev, oracle):
+    n = 100003
+    say("conversions", n)
+    x = np.concatenate([oracle.uniform(n, 3), oracle.loguniform(n, 3),
+                        np.array([0.0, -0.0, 1e-40, 3.0e38, 1 + 2.0 ** -9, 1 + 3 * 2.0 ** -9])])
+    dx = DeviceArray.from_numpy(x)
+    d32 = DeviceArray(x.size, np.float32)
+    check(libspx().spx_convert_f64_to_f32(x.size, dx.ptr, d32.ptr), "f64->f32")
+    assert np.array_equal(d32.to_numpy().view(np.uint32), x.astype(np.float32).view(np.uint32))
+    d16 = DeviceArray(x.size, np.uint16)
+    check(libspx().spx_convert_f64_to_bf16(x.size, dx.ptr, d16.ptr), "f64->bf16")
+    got = d16.to_numpy()
+
+    def val(code):
+        return (code.astype(np.uint32) << np.uint32(16)).view(np.float32).astype(np.float64)
+
+    # nearest: no bf16 neighbour of the result is closer to x; ties go to even
+    g = val(got)
+    up = val(got + np.uint16(1))
+    down_code = np.where((got & np.uint16(0x7FFF)) != 0, got - np.uint16(1), (got ^ np.uint16(0x8000)) + np.uint16(1))
+    down = val(down_code.astype(np.uint16))
+    dg, du, dd = np.abs(x - g), np.abs(x - up), np.abs(x - down)
+    assert np.all(dg <= du) and np.all(dg <= dd)
+    tie = (dg == du) | (dg == dd)
+    assert np.all((got[tie] & 1) == 0)
+    assert got[np.where(x == 0)[0][1]] == 0x8000  # -0.0 keeps its sign

@@ -1,0 +1,314 @@
+"""Pins of the CPU oracle (CPU only).
+
+The reference holds no golden vectors, known-answer tests or fixtures for
+this path (it never initialises or checks data).  Each oracle function is
+pinned here by something other than its own formula: big-int recurrences,
+exact integer arithmetic, hand-derived order probes whose expected values
+follow from IEEE round-to-nearest-even and fail for any other summation
+order, brute force on tiny meshes, and closed-form counts.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as o
+
+A, Cc, M = 6364136223846793005, 1, 1 << 64
+
+
+def lcg_python(seed, n):
+    x = (A * ((M - 1) ^ seed) + Cc) % M
+    out = []
+    for _ in range(n):
+        out.append(x)
+        x = (A * x + Cc) % M
+    return out
+
+
+# ------------------------------------------------------------------ LCG ----
+@pytest.mark.parametrize("seed", [0, 1, 2, 3, 4, 12345])
+def test_lcg_states_match_bigint_recurrence(seed):
+    xs = lcg_python(seed, 1100)
+    for k in (0, 1, 2, 7, 64, 511, 512, 1099):
+        assert o.lcg_state(seed, k) == xs[k]
+
+
+def test_uniform_mapping_exact():
+    xs = lcg_python(1, 2000)
+    v = o.uniform(2000, 1)
+    for x, got in zip(xs, v):
+        want = Fraction(-1) + Fraction(x >> 11, 1 << 52)
+        assert Fraction(got) == want
+    assert v.min() >= -1.0 and v.max() < 1.0
+    # continuing a stream = skipping ahead
+    assert np.array_equal(o.uniform(500, 1, 1500), v[1500:])
+
+
+def test_rand_int_is_abs_of_low_word():
+    xs = lcg_python(0, 1000)
+    got = o.rand_int(1000, 0)
+    for x, g in zip(xs, got):
+        w = x & 0xFFFFFFFF
+        s = w - (1 << 32) if w & 0x80000000 else w
+        assert g == abs(s)
+
+
+def test_loguniform_range_and_signs():
+    v = o.loguniform(20000, 3)
+    a = np.abs(v)
+    assert a.min() >= 1e-6 * (1 - 1e-12) and a.max() <= 1.0
+    assert 0.45 < np.mean(v < 0) < 0.55
+    # magnitude decades are uniform: each of the 6 decades holds ~1/6
+    dec = np.floor(-np.log10(a)).clip(0, 5)
+    counts = np.bincount(dec.astype(int), minlength=6) / a.size
+    assert np.all(np.abs(counts - 1 / 6) < 0.02)
+
+
+# ------------------------------------------------------------- BS1, BS2 ----
+def test_copy_is_identity():
+    a = o.uniform(12345, 1)
+    assert np.array_equal(o.bs1_copy(a).view(np.uint64), a.view(np.uint64))
+
+
+def test_axpy_exact_on_dyadic_inputs():
+    k = np.arange(-500, 500)
+    x, y = k / 8.0, (k[::-1] % 17) / 4.0
+    got = o.bs2_axpy(0.5, x, -2.0, y)
+    want = [float(Fraction(1, 2) * Fraction(a) - 2 * Fraction(b)) for a, b in zip(x, y)]
+    assert np.array_equal(got, np.array(want))
+
+
+def test_axpy_is_not_contracted():
+    # alpha*x = 1 + 2^-29 + 2^-60 rounds to 1 + 2^-29; beta*y = -(1 + 2^-29) exactly.
+    # Two roundings and one add give 0; a fused multiply-add would give 2^-60.
+    a = 1.0 + 2.0 ** -30
+    got = o.bs2_axpy(a, np.array([a]), -1.0, np.array([1.0 + 2.0 ** -29]))
+    assert got[0] == 0.0
+
+
+# -------------------------------------------------------- BS3..BS5 order ----
+def test_norm_integer_known_answer():
+    rng = np.random.default_rng(7)
+    a = rng.integers(-1000, 1000, size=300001).astype(np.float64)
+    assert o.bs3_norm(a) == float(sum(int(x) ** 2 for x in a))
+
+
+def test_dot_serial_chain_probe():
+    # N = 196608, bs = 256: Nblock = 256, stride 65536.  Thread (0,0) sums a[0], a[65536],
+    # a[131072] in that order: 2^53 + 1 -> 2^53 (tie to even), + 1 -> 2^53.  Any order that adds
+    # the two ones first gives 2^53 + 2.
+    n = 196608
+    a, b = np.zeros(n), np.ones(n)
+    a[0], a[65536], a[131072] = 2.0 ** 53, 1.0, 1.0
+    assert o.bs4_dot(a, b) == 2.0 ** 53
+
+
+def test_dot_block_tree_probe():
+    # N = 256: one block.  Tree step w=128 adds s[128] into s[0]: 1 + 2^53 -> 2^53; s[1] = 1
+    # survives to the last step s[0]+s[1] = 2^53 + 1 -> 2^53.  Left-to-right or pairwise sums
+    # give 1 + 1 + 2^53 = 2^53 + 2.
+    a, b = np.zeros(256), np.ones(256)
+    a[0], a[1], a[128] = 1.0, 1.0, 2.0 ** 53
+    assert o.bs4_dot(a, b) == 2.0 ** 53
+    assert sum([1.0, 1.0, 2.0 ** 53]) == 2.0 ** 53 + 2
+
+
+def test_second_stage_probe():
+    # N = 512, bs = 256: Nblock = 2 and thread t of block b holds a[t + 256 b].  Block 0's tree
+    # gives partial[0] = 2^53 (from a[128]); block 1 gives partial[1] = a[256] + a[257].
+    # Stage 2 adds them as s[0] + s[1]: 2^53 + 1 -> 2^53, but 2^53 + 2 is exact.
+    a, b = np.zeros(512), np.ones(512)
+    a[128], a[256] = 2.0 ** 53, 1.0
+    assert o.bs4_dot(a, b) == 2.0 ** 53
+    a[257] = 1.0
+    assert o.bs4_dot(a, b) == 2.0 ** 53 + 2
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 65536, 65537, 1000003])
+def test_reductions_close_to_exact(n):
+    a, b = o.uniform(n, 1), o.uniform(n, 1, n)
+    exact_norm = math.fsum(x * x for x in a.tolist()) if n < 70000 else o.sum_longdouble(a, None, 0)
+    assert abs(o.bs3_norm(a) - exact_norm) <= 1e-14 * abs(exact_norm)
+    exact_dot = math.fsum((a * b).tolist()) if n < 70000 else o.sum_longdouble(a, b, 1)
+    assert abs(o.bs4_dot(a, b) - exact_dot) <= 1e-13 * max(abs(exact_dot), 1e-300) + 1e-300
+
+
+def test_cg_update_closed_form_and_order():
+    n = 70000
+    Ap, r = o.uniform(n, 3), o.uniform(n, 3, n)
+    v, r2 = o.bs5_update(Ap, 0.37, r)
+    want_r = np.array([float(np.float64(x) + np.float64(0.37 * y)) for x, y in zip(r, Ap)])
+    assert np.array_equal(r2, want_r)
+    # the reduction is the BS3 order applied to the updated r
+    assert v == o.bs3_norm(r2)
+    v2, x2, r3 = o.bs5_full(0.37, Ap, r, r, Ap)
+    assert np.array_equal(x2, r + np.array([0.37 * y for y in Ap]))
+    assert np.array_equal(r3, Ap - np.array([0.37 * y for y in r]))
+    assert v2 == o.bs3_norm(r3)
+
+
+# ------------------------------------------------------------ hex ids ----
+def brute_hex_ids(nx, ny, nz, p):
+    nq = p + 1
+    Gx, Gy = nx * p + 1, ny * p + 1
+    ids = []
+    for ez in range(nz):
+        for ey in range(ny):
+            for ex in range(nx):
+                for k in range(nq):
+                    for j in range(nq):
+                        for i in range(nq):
+                            ids.append(1 + (ex * p + i) + (ey * p + j) * Gx + (ez * p + k) * Gx * Gy)
+    return np.array(ids, dtype=np.int64)
+
+
+@pytest.mark.parametrize("mesh", [(1, 1, 1, 1), (2, 1, 1, 2), (2, 3, 2, 3), (3, 3, 3, 1)])
+def test_hex_ids_brute_force(mesh):
+    nx, ny, nz, p = mesh
+    lex = o.hex_ids(*mesh)
+    assert np.array_equal(lex, brute_hex_ids(*mesh))
+    assert np.unique(lex).size == (nx * p + 1) * (ny * p + 1) * (nz * p + 1)
+    mn = o.hex_ids(*mesh, scheme="min")
+    # same grouping: the map lex -> min is a bijection of labels
+    pairs = np.unique(np.stack([lex, mn]), axis=1)
+    assert pairs.shape[1] == np.unique(lex).size == np.unique(mn).size
+    # min label = 1 + first local index of the group
+    first = {}
+    for l, g in enumerate(lex):
+        first.setdefault(g, l)
+    assert all(mn[l] == 1 + first[g] for l, g in enumerate(lex))
+
+
+# ------------------------------------------------------------ ogs setup ----
+def brute_ogs(ids):
+    """First-appearance numbering by dictionary (one rank, all groups positive or as given)."""
+    order, members = [], {}
+    for l, v in enumerate(ids):
+        if v == 0:
+            continue
+        a = abs(int(v))
+        if a not in members:
+            members[a] = []
+            order.append(a)
+        members[a].append(l)
+    pos = [a for a in order if any(ids[l] > 0 for l in members[a])]
+    neg = [a for a in order if a not in set(pos)]
+    rows = pos + neg
+    rs, ci = [0], []
+    for a in rows:
+        ci += members[a]
+        rs.append(len(ci))
+    return np.array(rs), np.array(ci), len(pos)
+
+
+@pytest.mark.parametrize("mesh", [(1, 1, 1, 1), (2, 2, 1, 2), (3, 2, 2, 3)])
+@pytest.mark.parametrize("scheme", ["lex", "min"])
+def test_ogs_setup_first_appearance(mesh, scheme):
+    ids = o.hex_ids(*mesh, scheme=scheme)
+    og = o.Ogs(ids, kind=1, unique=True)
+    rs, ci, npos = brute_ogs(np.abs(ids))
+    assert np.array_equal(og.rowStartsT, rs) and np.array_equal(og.colIdsT, ci)
+    assert og.NrowsN == og.NrowsT == npos
+    # unique: exactly one positive member per row, N-CSR = the positive members
+    flags = og.ids[og.colIdsT] > 0
+    per_row = np.add.reduceat(flags.astype(int), og.rowStartsT[:-1])
+    assert np.all(per_row == 1)
+    assert np.array_equal(og.colIdsN, og.colIdsT[flags])
+    # the id scheme does not change the operator
+    other = o.Ogs(o.hex_ids(*mesh, scheme="lex" if scheme == "min" else "min"))
+    assert np.array_equal(other.rowStartsT, og.rowStartsT) and np.array_equal(other.colIdsT, og.colIdsT)
+
+
+def test_ogs_setup_signed_flags_and_zeros():
+    rng = np.random.default_rng(3)
+    for trial in range(30):
+        n = int(rng.integers(1, 400))
+        ids = rng.integers(-40, 41, size=n).astype(np.int64)
+        og = o.Ogs(ids, kind=1, unique=False)
+        rs, ci, npos = brute_ogs(ids)
+        assert np.array_equal(og.rowStartsT, rs) and np.array_equal(og.colIdsT, ci)
+        assert og.NrowsN == npos
+        groups = {}
+        for v in ids[ids != 0]:
+            groups.setdefault(abs(int(v)), []).append(v > 0)
+        assert og.gather_defined == all(sum(g) == 1 for g in groups.values())
+        un = o.Ogs(ids, kind=0, unique=False)
+        assert np.array_equal(un.colIdsN, un.colIdsT)
+
+
+def test_ogs_setup_rejects_bad_kinds():
+    ids = np.arange(1, 10, dtype=np.int64)
+    assert o.Ogs(ids, kind=0, unique=True).error == 1
+    assert o.Ogs(ids, kind=2, unique=True).error == 1
+
+
+def multiplicity_counts(nx, ny, nz, p):
+    def axis(n):
+        interior_faces = n - 1
+        total = n * p + 1
+        return {1: total - interior_faces, 2: interior_faces}
+    out = {}
+    for mx, cx in axis(nx).items():
+        for my, cy in axis(ny).items():
+            for mz, cz in axis(nz).items():
+                out[mx * my * mz] = out.get(mx * my * mz, 0) + cx * cy * cz
+    return out
+
+
+@pytest.mark.parametrize("mesh", [(2, 2, 2, 1), (3, 2, 4, 2), (4, 4, 4, 3)])
+def test_gather_scatter_multiplicity_kat(mesh):
+    ids = o.hex_ids(*mesh)
+    og = o.Ogs(ids)
+    out = og.gather_scatter(np.ones(ids.size))
+    g = og.gather(np.ones(ids.size))
+    counts = {int(k): int(v) for k, v in zip(*np.unique(g, return_counts=True))}
+    assert counts == multiplicity_counts(*mesh)
+    assert set(np.unique(out)) <= {1.0, 2.0, 4.0, 8.0}
+
+
+def test_gather_invariants_and_composition():
+    ids = o.hex_ids(3, 2, 2, 3)
+    og = o.Ogs(ids)
+    q = np.floor(o.uniform(ids.size, 4) * 1000)  # integers: sums exact
+    g = og.gather(q, 1, 0, 2)
+    assert g.sum() == q.sum()
+    s = og.scatter(g, np.zeros(ids.size), 1, 1)
+    assert np.array_equal(s, og.gather_scatter(q))
+    qf = o.uniform(ids.size, 4)
+    assert np.array_equal(og.scatter(og.gather(qf, 1, 0, 2), np.zeros(ids.size), 1, 1), og.gather_scatter(qf))
+
+
+def test_order_probe_corner_row():
+    ids = o.hex_ids(2, 2, 2, 1)
+    og = o.Ogs(ids)
+    r = int(np.argmax(np.diff(og.rowStartsT) == 8))
+    m = og.colIdsT[og.rowStartsT[r]:og.rowStartsT[r + 1]]
+    q = np.zeros(ids.size)
+    q[m[0]], q[m[1]], q[m[2]] = 1e16, 1.0, -1e16
+    out = og.gather_scatter(q)
+    # ascending local order: (1e16 + 1) - 1e16 = 0 since 1e16 + 1 rounds to 1e16
+    assert np.all(out[m] == 0.0)
+    assert (1e16 - 1e16) + 1.0 == 1.0  # the other order would give 1
+
+
+def test_okl_block_form_agrees_and_blocks_valid():
+    ids = o.hex_ids(8, 8, 8, 7)
+    og = o.Ogs(ids)
+    bs, rs = og.blockStartsT, og.rowStartsT
+    assert bs[0] == 0 and bs[-1] == og.NrowsT and np.all(np.diff(bs) > 0)
+    assert np.all(rs[bs[1:]] - rs[bs[:-1]] <= 512)
+    q = o.uniform(ids.size, 0)
+    assert np.array_equal(o.gs_okl(bs, rs, og.colIdsT, rs, og.colIdsT, q), og.gather_scatter(q))
+
+
+def test_mul_max_min_ops():
+    ids = np.array([1, 2, 1, 3, 2, 1], dtype=np.int64)
+    og = o.Ogs(ids, kind=0, unique=False)
+    q = np.array([2.0, -3.0, 5.0, 7.0, 4.0, -1.0])
+    assert np.array_equal(og.gather_scatter(q, 1, 1, 0), [-10.0, -12.0, -10.0, 7.0, -12.0, -10.0])
+    assert np.array_equal(og.gather_scatter(q, 1, 2, 0), [5.0, 4.0, 5.0, 7.0, 4.0, 5.0])
+    assert np.array_equal(og.gather_scatter(q, 1, 3, 0), [-1.0, -3.0, -1.0, 7.0, -3.0, -1.0])
+    qi = np.array([3, 1, 4, 1, 5, 9], dtype=np.int32)
+    assert np.array_equal(og.gather_scatter(qi, 1, 0, 0), [16, 6, 16, 1, 6, 16])
