@@ -200,6 +200,8 @@ typedef struct {
   int32_t xcd_remap;     /* 1: map consecutive row blocks onto one XCD's L2 */
   int32_t unroll;        /* elements per thread per step in the vector kernels */
   int32_t grid_blocks;   /* fixed grid of the FAST reductions */
+  int32_t nontemporal;   /* 1: streaming (non-temporal) loads/stores in BS1/BS2 */
+  int32_t variant;       /* kernel variant of the family (0 = default; see the kernel source) */
 } spx_tuning_t;
 int spx_set_tuning(const char *kernel, const spx_tuning_t *tuning);
 int spx_get_tuning(const char *kernel, spx_tuning_t *tuning);

@@ -28,7 +28,9 @@ typedef struct {
   int32_t NlocalP;        /* gathered nodes with a positive member (ogsBase.hpp:50) */
   int32_t nnzT, nnzN;     /* CSR entries (ogsOperator.hpp:51-52) */
   int32_t NblocksT, NblocksN;
-  int32_t block_nnz;      /* LDS tile: max entries per row block */
+  int32_t block_nnz;      /* LDS tile of gather / scatter: max entries per row block */
+  int32_t gs_block_nnz;   /* LDS tile of gather+scatter */
+  int32_t Ngs_blocks;     /* row blocks of gather+scatter */
   int32_t gather_defined; /* ogsBase.hpp:57 */
   int32_t kind, unique;
   int64_t NgatherGlobal;
@@ -49,8 +51,10 @@ int spxh_ogs_info(spxh_ogs_t ogs, spxh_ogs_info_t *info);
 /* host copies of the operator: which = 0 (T-CSR) or 1 (N-CSR); arrays sized
    rows+1, nnz and blocks+1 (any may be NULL) */
 int spxh_ogs_copy_csr(spxh_ogs_t ogs, int which, int32_t *rowStarts, int32_t *colIds, int32_t *blockStarts);
-/* re-cut the row blocks for another LDS tile (autotuner) */
+/* re-cut the row blocks for another LDS tile (autotuner): gather / scatter
+   blocks, and gather+scatter blocks */
 int spxh_ogs_set_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
+int spxh_ogs_set_gs_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
 /* ogs_t::SetupGlobalToLocalMapping (ogsSetup.cpp:833-849) */
 int spxh_ogs_global_to_local(spxh_ogs_t ogs, int32_t *g2l);
 

@@ -30,12 +30,13 @@ class DeviceProps(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block_threads", C.c_int32), ("xcd_remap", C.c_int32), ("unroll", C.c_int32),
-                ("grid_blocks", C.c_int32)]
+                ("grid_blocks", C.c_int32), ("nontemporal", C.c_int32), ("variant", C.c_int32)]
 
 
 class OgsInfo(C.Structure):
     _fields_ = [("N", i32), ("Ngather", i32), ("NlocalT", i32), ("NlocalP", i32), ("nnzT", i32), ("nnzN", i32),
-                ("NblocksT", i32), ("NblocksN", i32), ("block_nnz", i32), ("gather_defined", i32),
+                ("NblocksT", i32), ("NblocksN", i32), ("block_nnz", i32), ("gs_block_nnz", i32),
+                ("Ngs_blocks", i32), ("gather_defined", i32),
                 ("kind", i32), ("unique", i32), ("NgatherGlobal", i64), ("setup_seconds", dbl)]
 
 
@@ -101,6 +102,7 @@ SPXH_FUNCS = {
     "spxh_ogs_info": (C.c_int, [vp, C.POINTER(OgsInfo)]),
     "spxh_ogs_copy_csr": (C.c_int, [vp, C.c_int, vp, vp, vp]),
     "spxh_ogs_set_block_nnz": (C.c_int, [vp, i32]),
+    "spxh_ogs_set_gs_block_nnz": (C.c_int, [vp, i32]),
     "spxh_ogs_global_to_local": (C.c_int, [vp, vp]),
     "spxh_ogs_gather_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "spxh_ogs_gather": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),

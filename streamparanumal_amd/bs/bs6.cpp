@@ -13,7 +13,7 @@ int main(int argc, char **argv) {
     bs::addVerbose(settings);
     bs::addSeed(settings);
     settings.newSetting("--fields", "--fields", "FIELDS", "1", "Fields per node (K)");
-    settings.newSetting("--block-nnz", "--block-nnz", "BLOCK NNZ", "2048", "Max column entries per row block");
+    settings.newSetting("--block-nnz", "--block-nnz", "BLOCK NNZ", "0", "Max column entries per row block (0: library default)");
     settings.parseSettings(argc, argv);
     if (settings.helpRequested()) return;
 
@@ -24,13 +24,16 @@ int main(int argc, char **argv) {
       meshReportSettings(settings);
     }
     mesh_t mesh(platform, settings, settings.comm);
-    int K = 1, blockNnz = 2048;
+    int K = 1, blockNnz = 0;
     unsigned long long seed = 1;
     settings.getSetting("FIELDS", K);
     settings.getSetting("BLOCK NNZ", blockNnz);
     settings.getSetting("SEED", seed);
     LIBP_ABORT("--fields must be >= 1", K < 1);
-    if (blockNnz != mesh.ogs.localOperator()->blockNnz) mesh.ogs.localOperator()->SetBlockNnz(blockNnz);
+    if (blockNnz > 0) {
+      mesh.ogs.localOperator()->SetBlockNnz(blockNnz);
+      mesh.ogs.localOperator()->SetGsBlockNnz(blockNnz);
+    }
 
     const dlong N = mesh.Np * mesh.Nelements;
     const dlong Ngather = mesh.ogs.Ngather;

@@ -1,0 +1,92 @@
+// Launchers of the ogs kernels for one value type, SPX_OGS_T (compiled once
+// per type so the kernel instances build in parallel).
+#include "spx_ogs_kernels.hpp"
+
+#ifndef SPX_OGS_T
+#error "compile with -DSPX_OGS_T=<float|double|int|long long>"
+#endif
+
+namespace spx {
+namespace ogsk {
+
+using T = SPX_OGS_T;
+
+// Calls F.template operator()<OPC, KF, PER>() for the runtime (op, kf, per).
+template <bool WITH_OP, typename F>
+static int dispatch(int op, int kf, int per, F &&f) {
+  auto by_shape = [&](auto opc) -> int {
+    constexpr int OPC = decltype(opc)::value;
+    if (kf == 1 && per == 4) return f.template operator()<OPC, 1, 4>();
+    if (kf == 1 && per == 8) return f.template operator()<OPC, 1, 8>();
+    if (kf == 1 && per == 16) return f.template operator()<OPC, 1, 16>();
+    if (kf == 2 && per == 4) return f.template operator()<OPC, 2, 4>();
+    if (kf == 2 && per == 8) return f.template operator()<OPC, 2, 8>();
+    if (kf == 3 && per == 4) return f.template operator()<OPC, 3, 4>();
+    if (kf == 4 && per == 4) return f.template operator()<OPC, 4, 4>();
+    return fail(SPX_ERR_UNSUPPORTED, "ogs: no kernel for fields-per-pass " + std::to_string(kf) +
+                                         " with " + std::to_string(per) + " entries per lane");
+  };
+  if (!WITH_OP) return by_shape(std::integral_constant<int, SPX_ADD>{});
+  switch (op) {
+    case SPX_ADD: return by_shape(std::integral_constant<int, SPX_ADD>{});
+    case SPX_MUL: return by_shape(std::integral_constant<int, SPX_MUL>{});
+    case SPX_MAX: return by_shape(std::integral_constant<int, SPX_MAX>{});
+    default: return by_shape(std::integral_constant<int, SPX_MIN>{});
+  }
+}
+
+static dim3 grid_of(const Launch &L) { return dim3(L.nblocks, L.kf == 1 ? L.K : 1); }
+
+int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const int *ids, T *q) {
+  const size_t rsb = L.variant == 1 ? (size_t)(L.cap + 1) * sizeof(int)
+                    : L.variant == 3 ? (size_t)(L.cap + 1) * sizeof(unsigned short) : 0;
+  const size_t lds = lds_bytes(L.cap, L.kf, sizeof(T), 1) + rsb;
+  if (lds > 65536) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter: tile too large for LDS");
+  return dispatch<true>(op, L.kf, L.per, [&]<int OPC, int KF, int PER>() {
+    if (L.variant == 1)
+      hipLaunchKernelGGL((k_gs_sym<T, OPC, KF, PER, 1>), grid_of(L), dim3(kThreads), lds, L.stream, L, bs, st, ids, q);
+    else if (L.variant == 2)
+      hipLaunchKernelGGL((k_gs_sym<T, OPC, KF, PER, 2>), grid_of(L), dim3(kThreads), lds, L.stream, L, bs, st, ids, q);
+    else if (L.variant == 3)
+      hipLaunchKernelGGL((k_gs_sym<T, OPC, KF, PER, 3>), grid_of(L), dim3(kThreads), lds, L.stream, L, bs, st, ids, q);
+    else
+      hipLaunchKernelGGL((k_gs_sym<T, OPC, KF, PER, 0>), grid_of(L), dim3(kThreads), lds, L.stream, L, bs, st, ids, q);
+    return SPX_OK;
+  });
+}
+
+int launch_gs_gen(const Launch &L, int op, int ngrows, int nsrows, const int *bs, const int *gs, const int *gi,
+                  const int *ss, const int *si, T *q) {
+  const size_t lds = lds_bytes(L.cap, L.kf, sizeof(T), 2);
+  return dispatch<true>(op, L.kf, L.per, [&]<int OPC, int KF, int PER>() {
+    hipLaunchKernelGGL((k_gs_gen<T, OPC, KF, PER>), grid_of(L), dim3(kThreads), lds, L.stream, L, ngrows, nsrows,
+                       bs, gs, gi, ss, si, q);
+    return SPX_OK;
+  });
+}
+
+int launch_gather(const Launch &L, int op, const int *bs, const int *st, const int *ids, const T *v, T *gv) {
+  const size_t lds = lds_bytes(L.cap, L.kf, sizeof(T), 1);
+  return dispatch<true>(op, L.kf, L.per, [&]<int OPC, int KF, int PER>() {
+    hipLaunchKernelGGL((k_gather<T, OPC, KF, PER>), grid_of(L), dim3(kThreads), lds, L.stream, L, bs, st, ids, v, gv);
+    return SPX_OK;
+  });
+}
+
+int launch_scatter(const Launch &L, const int *bs, const int *st, const int *ids, const T *gv, T *v) {
+  const size_t lds = lds_bytes(L.cap, L.kf, sizeof(T), 1);
+  return dispatch<false>(SPX_ADD, L.kf, L.per, [&]<int OPC, int KF, int PER>() {
+    hipLaunchKernelGGL((k_scatter<T, KF, PER>), grid_of(L), dim3(kThreads), lds, L.stream, L, bs, st, ids, gv, v);
+    return SPX_OK;
+  });
+}
+
+int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq) {
+  int64_t g = (total + kThreads - 1) / kThreads;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_extract<T>, dim3((unsigned)g), dim3(kThreads), 0, s, total, K, ids, q, gq);
+  return SPX_OK;
+}
+
+}  // namespace ogsk
+}  // namespace spx

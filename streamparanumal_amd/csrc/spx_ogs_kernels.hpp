@@ -1,0 +1,433 @@
+// ogs gather / scatter / gather+scatter kernels for gfx950.
+//
+// Same operator as libs/ogs/okl/ogsKernels.okl (a CSR with one row per
+// gathered node and the local indices of its copies as columns), re-designed
+// for CDNA:
+//   * one workgroup (256 lanes) per row block; a block holds whole rows and at
+//     most 256*PER column entries, staged through LDS sized at launch;
+//   * phase 1 is straight-line code: each lane issues PER column-id loads,
+//     PER+1 row-start loads and then PER value loads (indices clamped into the
+//     block), so every load of the phase is in flight before the first use
+//     and the wait counters stay exact;
+//   * phase 2: one lane per row folds its entries in stored order (ascending
+//     local index) out of LDS, so sums are bit-identical to the reference's
+//     Serial order;
+//   * phase 3: lanes write results back in CSR order;
+//   * no atomics: every local node belongs to exactly one row and every row to
+//     exactly one block, so the in-place update is race-free;
+//   * workgroup ids are remapped so that XCD x owns one contiguous range of
+//     row blocks (neighbouring blocks share cache lines at their edges).
+// K fields per node (AoS, stride K) are processed together (KF = K <= 4) or
+// one field per grid row (KF = 1, grid.y = K), like the reference's
+// @outer(1) loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "spx_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace spx {
+namespace ogsk {
+
+constexpr int kThreads = 256;
+
+struct Launch {
+  int nblocks;
+  int K;      // fields per node (stride)
+  int nxcd;   // XCD remap width (1 = identity)
+  int cap;    // LDS capacity in entries (>= every block's nnz)
+  int kf;     // fields per pass
+  int per;    // entries per lane (cap <= 256*per)
+  int variant;  // Sym GS: 0 row ranges in registers, 1 row starts in LDS, 2 = 0 + nontemporal write-back,
+                // 3 = 16-bit row starts in LDS
+  hipStream_t stream;
+  int *err;
+};
+
+template <typename T> struct Lim;
+template <> struct Lim<float> { static __device__ __forceinline__ float max() { return FLT_MAX; } };
+template <> struct Lim<double> { static __device__ __forceinline__ double max() { return DBL_MAX; } };
+template <> struct Lim<int> { static __device__ __forceinline__ int max() { return INT_MAX; } };
+template <> struct Lim<long long> { static __device__ __forceinline__ long long max() { return LLONG_MAX; } };
+
+// ogsOperator.cpp:37-56 (host) / ogsUtils.cpp:62-103 (device defines)
+template <typename T, int OP> struct Fold;
+template <typename T> struct Fold<T, SPX_ADD> {
+  static __device__ __forceinline__ T init() { return T(0); }
+  static __device__ __forceinline__ void apply(T &a, T b) { a += b; }
+};
+template <typename T> struct Fold<T, SPX_MUL> {
+  static __device__ __forceinline__ T init() { return T(1); }
+  static __device__ __forceinline__ void apply(T &a, T b) { a *= b; }
+};
+template <typename T> struct Fold<T, SPX_MAX> {
+  static __device__ __forceinline__ T init() { return -Lim<T>::max(); }
+  static __device__ __forceinline__ void apply(T &a, T b) { a = (b > a) ? b : a; }
+};
+template <typename T> struct Fold<T, SPX_MIN> {
+  static __device__ __forceinline__ T init() { return Lim<T>::max(); }
+  static __device__ __forceinline__ void apply(T &a, T b) { a = (b < a) ? b : a; }
+};
+
+__device__ __forceinline__ void flag_error(int *err, int code) {
+  if (threadIdx.x == 0) err[0] = code;
+}
+
+__device__ __forceinline__ int clampi(int x, int hi) { return x < hi ? x : hi; }
+
+// Column ids of entries t, t+256, ... (clamped to the block's last entry).
+template <int PER>
+__device__ __forceinline__ void load_ids(int (&cid)[PER], const int *__restrict__ ids, int g0, int nnz) {
+  const int last = nnz > 0 ? nnz - 1 : 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) cid[j] = ids[g0 + clampi(threadIdx.x + j * kThreads, last)];
+}
+
+// Relative row starts rs[r] = starts[r0 + min(r, rlim)] - base for r in [0, nrows],
+// staged through registers so the loads overlap the id loads.
+template <int PER>
+__device__ __forceinline__ void stage_starts(int *rs, const int *__restrict__ starts, int r0, int nrows,
+                                             int rlim, int base) {
+  int x[PER + 1];
+#pragma unroll
+  for (int j = 0; j <= PER; ++j) x[j] = starts[r0 + clampi(clampi(threadIdx.x + j * kThreads, nrows), rlim)];
+#pragma unroll
+  for (int j = 0; j <= PER; ++j)
+    if (threadIdx.x + j * kThreads <= nrows) rs[threadIdx.x + j * kThreads] = x[j] - base;
+}
+
+// As stage_starts, into 16-bit LDS offsets (a block holds at most 4096 entries).
+template <int PER>
+__device__ __forceinline__ void stage_starts16(unsigned short *rs, const int *__restrict__ starts, int r0,
+                                               int nrows, int base) {
+  int x[PER + 1];
+#pragma unroll
+  for (int j = 0; j <= PER; ++j) x[j] = starts[r0 + clampi(threadIdx.x + j * kThreads, nrows)];
+#pragma unroll
+  for (int j = 0; j <= PER; ++j)
+    if (threadIdx.x + j * kThreads <= nrows) rs[threadIdx.x + j * kThreads] = (unsigned short)(x[j] - base);
+}
+
+// Row ranges of the rows a lane folds (rows t, t+256, ...), relative to g0,
+// loaded straight into registers: ra[j] = start, rb[j] = end.
+template <int PER>
+__device__ __forceinline__ void load_row_ranges(int (&ra)[PER], int (&rb)[PER], const int *__restrict__ starts,
+                                                int r0, int nrows, int g0) {
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int r = r0 + clampi(threadIdx.x + j * kThreads, nrows);
+    ra[j] = starts[r] - g0;
+    rb[j] = starts[clampi(r + 1, r0 + nrows)] - g0;
+  }
+}
+
+// ---------------------------------------------------------------- Sym GS --
+// gatherScatter with gStarts == sStarts, gIds == sIds (ogs::Sym).
+// LDS: vals[cap*KF] of T (+ rs[cap+1] ints for VAR 1).
+//   VAR 0: each lane keeps the ranges of its rows in registers;
+//   VAR 1: the block's row starts are staged in LDS;
+//   VAR 2: as 0, with non-temporal write-back of the results;
+//   VAR 3: as 1, with 16-bit row starts.
+template <typename T, int OP, int KF, int PER, int VAR>
+__global__ __launch_bounds__(kThreads) void k_gs_sym(Launch L, const int *__restrict__ blockStarts,
+                                                     const int *__restrict__ starts,
+                                                     const int *__restrict__ ids, T *__restrict__ q) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *vals = reinterpret_cast<T *>(lds_raw);
+  int *rs = reinterpret_cast<int *>(vals + (size_t)L.cap * KF);
+  unsigned short *rs16 = reinterpret_cast<unsigned short *>(rs);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int kb = blockIdx.y * KF;
+  const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
+  const int nrows = r1 - r0;
+  const int g0 = starts[r0];
+  const int nnz = starts[r1] - g0;
+  if (nnz > L.cap || nrows > L.cap) {
+    flag_error(L.err, 1);
+    return;
+  }
+  if (nnz == 0) return;
+  const int64_t K = L.K;
+  const int t = threadIdx.x;
+  int cid[PER], ra[PER], rb[PER];
+  load_ids<PER>(cid, ids, g0, nnz);
+  if constexpr (VAR == 1) stage_starts<PER>(rs, starts, r0, nrows, nrows, g0);
+  else if constexpr (VAR == 3) stage_starts16<PER>(rs16, starts, r0, nrows, g0);
+  else load_row_ranges<PER>(ra, rb, starts, r0, nrows, g0);
+  T v[PER][KF];
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+#pragma unroll
+    for (int f = 0; f < KF; ++f) v[j][f] = q[(int64_t)cid[j] * K + kb + f];
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * kThreads < nnz)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) vals[(t + j * kThreads) * KF + f] = v[j][f];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * kThreads < nrows) {
+      int a, e;
+      if constexpr (VAR == 1) {
+        a = rs[t + j * kThreads];
+        e = rs[t + j * kThreads + 1];
+      } else if constexpr (VAR == 3) {
+        a = rs16[t + j * kThreads];
+        e = rs16[t + j * kThreads + 1];
+      } else {
+        a = ra[j];
+        e = rb[j];
+      }
+#pragma unroll
+      for (int f = 0; f < KF; ++f) {
+        T acc = Fold<T, OP>::init();
+        for (int i = a; i < e; ++i) Fold<T, OP>::apply(acc, vals[i * KF + f]);
+        for (int i = a; i < e; ++i) vals[i * KF + f] = acc;
+      }
+    }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * kThreads < nnz)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) {
+        T *dst = q + (int64_t)cid[j] * K + kb + f;
+        const T x = vals[(t + j * kThreads) * KF + f];
+        if constexpr (VAR == 2) __builtin_nontemporal_store(x, dst);
+        else *dst = x;
+      }
+}
+
+// ------------------------------------------------------------ general GS --
+// gather CSR (gStarts/gIds, ngrows rows) and scatter CSR (sStarts/sIds,
+// nsrows rows) differ (ogs::NoTrans, ogs::Trans).  Rows at or past ngrows
+// gather nothing (their value is the op's identity); rows at or past nsrows
+// scatter nothing.  LDS: gvals[cap*KF], svals[cap*KF], grs[cap+1], srs[cap+1].
+template <typename T, int OP, int KF, int PER>
+__global__ __launch_bounds__(kThreads) void k_gs_gen(Launch L, int ngrows, int nsrows,
+                                                     const int *__restrict__ blockStarts,
+                                                     const int *__restrict__ gStarts,
+                                                     const int *__restrict__ gIds,
+                                                     const int *__restrict__ sStarts,
+                                                     const int *__restrict__ sIds, T *__restrict__ q) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *gvals = reinterpret_cast<T *>(lds_raw);
+  T *svals = gvals + (size_t)L.cap * KF;
+  int *grs = reinterpret_cast<int *>(svals + (size_t)L.cap * KF);
+  int *srs = grs + L.cap + 1;
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int kb = blockIdx.y * KF;
+  const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
+  const int nrows = r1 - r0;
+  const int gr0 = clampi(r0, ngrows), gr1 = clampi(r1, ngrows);
+  const int sr0 = clampi(r0, nsrows), sr1 = clampi(r1, nsrows);
+  const int g0 = gStarts[gr0];
+  const int gnnz = gStarts[gr1] - g0;
+  const int s0 = sStarts[sr0];
+  const int snnz = sStarts[sr1] - s0;
+  if (gnnz > L.cap || snnz > L.cap || nrows > L.cap) {
+    flag_error(L.err, 2);
+    return;
+  }
+  const int64_t K = L.K;
+  const int t = threadIdx.x;
+  // rows past a CSR's end read that CSR's last start: empty ranges
+  if (gr0 == r0) stage_starts<PER>(grs, gStarts, r0, nrows, gr1 - r0, g0);
+  else
+    for (int r = t; r <= nrows; r += kThreads) grs[r] = 0;
+  if (sr0 == r0) stage_starts<PER>(srs, sStarts, r0, nrows, sr1 - r0, s0);
+  else
+    for (int r = t; r <= nrows; r += kThreads) srs[r] = 0;
+  if (gnnz > 0) {
+    int cid[PER];
+    load_ids<PER>(cid, gIds, g0, gnnz);
+    T v[PER][KF];
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) v[j][f] = q[(int64_t)cid[j] * K + kb + f];
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (t + j * kThreads < gnnz)
+#pragma unroll
+        for (int f = 0; f < KF; ++f) gvals[(t + j * kThreads) * KF + f] = v[j][f];
+  }
+  __syncthreads();
+  for (int r = t; r < nrows; r += kThreads) {
+    const int a = grs[r], e = grs[r + 1], sa = srs[r], se = srs[r + 1];
+#pragma unroll
+    for (int f = 0; f < KF; ++f) {
+      T acc = Fold<T, OP>::init();
+      for (int i = a; i < e; ++i) Fold<T, OP>::apply(acc, gvals[i * KF + f]);
+      for (int i = sa; i < se; ++i) svals[i * KF + f] = acc;
+    }
+  }
+  __syncthreads();
+  if (snnz > 0) {
+    int cid[PER];
+    load_ids<PER>(cid, sIds, s0, snnz);
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (t + j * kThreads < snnz)
+#pragma unroll
+        for (int f = 0; f < KF; ++f) q[(int64_t)cid[j] * K + kb + f] = svals[(t + j * kThreads) * KF + f];
+  }
+}
+
+// ---------------------------------------------------------------- gather --
+// LDS: vals[cap*KF].  Row ranges live in registers.
+template <typename T, int OP, int KF, int PER>
+__global__ __launch_bounds__(kThreads) void k_gather(Launch L, const int *__restrict__ blockStarts,
+                                                     const int *__restrict__ starts,
+                                                     const int *__restrict__ ids,
+                                                     const T *__restrict__ v, T *__restrict__ gv) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *vals = reinterpret_cast<T *>(lds_raw);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int kb = blockIdx.y * KF;
+  const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
+  const int nrows = r1 - r0;
+  const int g0 = starts[r0];
+  const int nnz = starts[r1] - g0;
+  if (nnz > L.cap || nrows > L.cap) {
+    flag_error(L.err, 3);
+    return;
+  }
+  if (nrows == 0) return;
+  const int64_t K = L.K;
+  const int t = threadIdx.x;
+  int ra[PER], rb[PER];
+  load_row_ranges<PER>(ra, rb, starts, r0, nrows, g0);
+  if (nnz > 0) {
+    int cid[PER];
+    load_ids<PER>(cid, ids, g0, nnz);
+    T x[PER][KF];
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) x[j][f] = v[(int64_t)cid[j] * K + kb + f];
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (t + j * kThreads < nnz)
+#pragma unroll
+        for (int f = 0; f < KF; ++f) vals[(t + j * kThreads) * KF + f] = x[j][f];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * kThreads < nrows) {
+      const int a = ra[j], e = rb[j];
+#pragma unroll
+      for (int f = 0; f < KF; ++f) {
+        T acc = Fold<T, OP>::init();
+        for (int i = a; i < e; ++i) Fold<T, OP>::apply(acc, vals[i * KF + f]);
+        gv[(int64_t)(r0 + t + j * kThreads) * K + kb + f] = acc;
+      }
+    }
+}
+
+// --------------------------------------------------------------- scatter --
+// LDS: vals[cap*KF].  Row ranges and row values live in registers.
+template <typename T, int KF, int PER>
+__global__ __launch_bounds__(kThreads) void k_scatter(Launch L, const int *__restrict__ blockStarts,
+                                                      const int *__restrict__ starts,
+                                                      const int *__restrict__ ids,
+                                                      const T *__restrict__ gv, T *__restrict__ v) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *vals = reinterpret_cast<T *>(lds_raw);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int kb = blockIdx.y * KF;
+  const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
+  const int nrows = r1 - r0;
+  const int g0 = starts[r0];
+  const int nnz = starts[r1] - g0;
+  if (nnz > L.cap || nrows > L.cap) {
+    flag_error(L.err, 4);
+    return;
+  }
+  if (nrows == 0 || nnz == 0) return;
+  const int64_t K = L.K;
+  const int t = threadIdx.x;
+  int cid[PER], ra[PER], rb[PER];
+  load_ids<PER>(cid, ids, g0, nnz);
+  load_row_ranges<PER>(ra, rb, starts, r0, nrows, g0);
+  T x[PER][KF];
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+#pragma unroll
+    for (int f = 0; f < KF; ++f) x[j][f] = gv[(int64_t)(r0 + clampi(t + j * kThreads, nrows - 1)) * K + kb + f];
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * kThreads < nrows) {
+      const int a = ra[j], e = rb[j];
+#pragma unroll
+      for (int f = 0; f < KF; ++f)
+        for (int i = a; i < e; ++i) vals[i * KF + f] = x[j][f];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * kThreads < nnz)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) v[(int64_t)cid[j] * K + kb + f] = vals[(t + j * kThreads) * KF + f];
+}
+
+// --------------------------------------------------------------- extract --
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_extract(int64_t total, int K, const int *__restrict__ ids,
+                                                      const T *__restrict__ q, T *__restrict__ gq) {
+  int64_t n = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * kThreads;
+  for (; n < total; n += st) {
+    const int64_t g = n / K;
+    const int k = (int)(n - g * K);
+    gq[n] = q[k + (int64_t)ids[g] * K];
+  }
+}
+
+// ------------------------------------------------------------- launching --
+// Supported (KF, PER) pairs; cap <= 256*PER.
+//   KF = 1: PER in {4, 8, 16}   (cap up to 1024 / 2048 / 4096)
+//   KF = 2: PER in {4, 8}
+//   KF = 3, 4: PER = 4
+inline int pick_per(int kf, int cap) {
+  const int need = (cap + kThreads - 1) / kThreads;
+  if (need <= 4) return 4;
+  if (need <= 8 && kf <= 2) return 8;
+  if (need <= 16 && kf == 1) return 16;
+  return 0;
+}
+
+// LDS per workgroup: one value tile (gather, scatter, Sym GS), or two value
+// tiles plus two row-start arrays (general GS).  Sym GS variant 1 adds its
+// row starts (launch_gs_sym).
+inline size_t lds_bytes(int cap, int kf, size_t tsize, int arrays) {
+  if (arrays == 1) return (size_t)cap * kf * tsize;
+  return 2 * ((size_t)cap * kf * tsize + (size_t)(cap + 1) * sizeof(int));
+}
+
+// Per-type launchers, defined by one instantiation unit per type.
+int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const int *ids, float *q);
+int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const int *ids, double *q);
+int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const int *ids, int *q);
+int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const int *ids, long long *q);
+#define SPX_OGS_GEN_DECL(T)                                                                                  \
+  int launch_gs_gen(const Launch &L, int op, int ngrows, int nsrows, const int *bs, const int *gs,          \
+                    const int *gi, const int *ss, const int *si, T *q);                                    \
+  int launch_gather(const Launch &L, int op, const int *bs, const int *st, const int *ids, const T *v, T *gv); \
+  int launch_scatter(const Launch &L, const int *bs, const int *st, const int *ids, const T *gv, T *v);    \
+  int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq);
+SPX_OGS_GEN_DECL(float)
+SPX_OGS_GEN_DECL(double)
+SPX_OGS_GEN_DECL(int)
+SPX_OGS_GEN_DECL(long long)
+#undef SPX_OGS_GEN_DECL
+
+}  // namespace ogsk
+}  // namespace spx

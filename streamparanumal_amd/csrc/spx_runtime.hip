@@ -49,6 +49,8 @@ spx_tuning_t tuning_for(const char *kernel, spx_tuning_t d) {
   if (t.xcd_remap < 0) t.xcd_remap = d.xcd_remap;
   if (t.unroll <= 0) t.unroll = d.unroll;
   if (t.grid_blocks <= 0) t.grid_blocks = d.grid_blocks;
+  if (t.nontemporal < 0) t.nontemporal = d.nontemporal;
+  if (t.variant < 0) t.variant = d.variant;
   return t;
 }
 
@@ -197,6 +199,7 @@ int spx_device_props(spx_device_props_t *out) {
   SPX_HIP(hipGetDeviceProperties(&p, g_dev.id));
   std::memset(out, 0, sizeof(*out));
   std::strncpy(out->name, p.name, sizeof(out->name) - 1);
+  if (!out->name[0]) (void)hipDeviceGetName(out->name, (int)sizeof(out->name) - 1, g_dev.id);
   std::strncpy(out->arch, p.gcnArchName, sizeof(out->arch) - 1);
   out->device_id = g_dev.id;
   out->compute_units = p.multiProcessorCount;
@@ -416,7 +419,7 @@ int spx_set_tuning(const char *kernel, const spx_tuning_t *t) {
   if (t->block_threads != 0 &&
       (t->block_threads < 64 || t->block_threads > 1024 || t->block_threads % 64))
     return fail(SPX_ERR_ARG, "spx_set_tuning: block_threads must be a multiple of 64 in [64,1024]");
-  if (t->unroll < 0 || t->unroll > 16) return fail(SPX_ERR_ARG, "spx_set_tuning: unroll in [0,16]");
+  if (t->unroll < 0 || t->unroll > 32) return fail(SPX_ERR_ARG, "spx_set_tuning: unroll in [0,32]");
   if (t->grid_blocks < 0) return fail(SPX_ERR_ARG, "spx_set_tuning: grid_blocks >= 0");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[kernel] = *t;
@@ -430,6 +433,8 @@ int spx_get_tuning(const char *kernel, spx_tuning_t *t) {
   if (it == g_tune.end()) {
     std::memset(t, 0, sizeof(*t));
     t->xcd_remap = -1;
+    t->nontemporal = -1;
+    t->variant = -1;
     return SPX_OK;
   }
   *t = it->second;

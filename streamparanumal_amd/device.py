@@ -156,8 +156,9 @@ class Graph:
             pass
 
 
-def set_tuning(kernel, block_threads=0, xcd_remap=-1, unroll=0, grid_blocks=0):
-    t = _lib.Tuning(block_threads, xcd_remap, unroll, grid_blocks)
+def set_tuning(kernel, block_threads=0, xcd_remap=-1, unroll=0, grid_blocks=0, nontemporal=-1, variant=-1):
+    """Launch parameters of one kernel family (0 / -1 keep the library default)."""
+    t = _lib.Tuning(block_threads, xcd_remap, unroll, grid_blocks, nontemporal, variant)
     check(libspx().spx_set_tuning(kernel.encode(), C.byref(t)), "spx_set_tuning")
 
 

@@ -9,10 +9,13 @@
 namespace libp {
 namespace ogs {
 
-// Reference defaults (ogsOperator.hpp:37-38) and this build's default tile.
+// Reference defaults (ogsOperator.hpp:37-38) and this build's default tiles,
+// measured on MI355X at 64^3 hex p=7: gather/scatter fastest with 2048-entry
+// row blocks, gather+scatter with 4096.
 constexpr int blockSize = 256;
 constexpr int gatherNodesPerBlock = 512;
 constexpr int defaultBlockNnz = 2048;
+constexpr int defaultGsBlockNnz = 4096;
 constexpr int maxBlockNnz = 4096;
 
 class ogsOperator_t {
@@ -28,10 +31,15 @@ class ogsOperator_t {
   memory<dlong> rowStartsN, rowStartsT, colIdsN, colIdsT;
   deviceMemory<dlong> o_rowStartsN, o_rowStartsT, o_colIdsN, o_colIdsT;
 
-  dlong blockNnz = defaultBlockNnz;  // LDS tile (max column entries per block)
+  dlong blockNnz = defaultBlockNnz;  // LDS tile of gather / scatter (max column entries per block)
   dlong NrowBlocksN = 0, NrowBlocksT = 0;
   memory<dlong> blockRowStartsN, blockRowStartsT;
   deviceMemory<dlong> o_blockRowStartsN, o_blockRowStartsT;
+
+  dlong gsBlockNnz = defaultGsBlockNnz;  // LDS tile of gather+scatter (row blocks of the T-CSR)
+  dlong NgsBlocks = 0;
+  memory<dlong> gsBlockRowStarts;
+  deviceMemory<dlong> o_gsBlockRowStarts;
 
   Kind kind = Signed;
 
@@ -39,11 +47,14 @@ class ogsOperator_t {
   // CSR arrays are taken as built (rows sorted, columns ascending in a row).
   ogsOperator_t(platform_t &platform_, Kind kind_, const dlong NrowsN_, const dlong NrowsT_,
                 const dlong Ncols_, memory<dlong> rowStartsT_, memory<dlong> colIdsT_,
-                memory<dlong> rowStartsN_, memory<dlong> colIdsN_, int blockNnz_ = defaultBlockNnz);
+                memory<dlong> rowStartsN_, memory<dlong> colIdsN_, int blockNnz_ = defaultBlockNnz,
+                int gsBlockNnz_ = defaultGsBlockNnz);
 
   void Free();
-  // Re-cut the row blocks for another LDS tile size (autotuner).
+  // Re-cut the row blocks for other LDS tile sizes (autotuner): gather and
+  // scatter blocks, and gather+scatter blocks.
   void SetBlockNnz(int nnz);
+  void SetGsBlockNnz(int nnz);
 
   template <typename T> void Gather(deviceMemory<T> gv, const deviceMemory<T> v, const int k, const Op op, const Transpose trans);
   template <typename T> void Scatter(deviceMemory<T> v, const deviceMemory<T> gv, const int k, const Transpose trans);

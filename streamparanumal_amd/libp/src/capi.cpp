@@ -101,6 +101,8 @@ int spxh_ogs_info(spxh_ogs_t h, spxh_ogs_info_t *info) {
     info->NblocksT = op->NrowBlocksT;
     info->NblocksN = op->NrowBlocksN;
     info->block_nnz = op->blockNnz;
+    info->gs_block_nnz = op->gsBlockNnz;
+    info->Ngs_blocks = op->NgsBlocks;
     info->gather_defined = o.gather_defined ? 1 : 0;
     info->kind = (int)o.kind;
     info->unique = o.unique ? 1 : 0;
@@ -134,6 +136,15 @@ int spxh_ogs_set_block_nnz(spxh_ogs_t h, int32_t nnz) {
     auto op = h->ogs.localOperator();
     LIBP_ABORT("ogs not set up", !op);
     op->SetBlockNnz(nnz);
+  });
+}
+
+int spxh_ogs_set_gs_block_nnz(spxh_ogs_t h, int32_t nnz) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_block_nnz: null");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    op->SetGsBlockNnz(nnz);
   });
 }
 

@@ -30,7 +30,8 @@ void blockRowsGreedy(const dlong Nrows, const memory<dlong> rowStarts, const int
 
 ogsOperator_t::ogsOperator_t(platform_t &platform_, Kind kind_, const dlong NrowsN_, const dlong NrowsT_,
                              const dlong Ncols_, memory<dlong> rowStartsT_, memory<dlong> colIdsT_,
-                             memory<dlong> rowStartsN_, memory<dlong> colIdsN_, int blockNnz_)
+                             memory<dlong> rowStartsN_, memory<dlong> colIdsN_, int blockNnz_,
+                             int gsBlockNnz_)
     : platform(platform_), Ncols(Ncols_), NrowsN(NrowsN_), NrowsT(NrowsT_), kind(kind_) {
   rowStartsT = rowStartsT_;
   colIdsT = colIdsT_;
@@ -51,6 +52,14 @@ ogsOperator_t::ogsOperator_t(platform_t &platform_, Kind kind_, const dlong Nrow
     o_colIdsN = o_colIdsT;
   }
   SetBlockNnz(blockNnz_);
+  SetGsBlockNnz(gsBlockNnz_);
+}
+
+void ogsOperator_t::SetGsBlockNnz(int nnz) {
+  LIBP_ABORT("row block capacity must be in [1, " << maxBlockNnz << "]", nnz < 1 || nnz > maxBlockNnz);
+  gsBlockNnz = nnz;
+  blockRowsGreedy(NrowsT, rowStartsT, gsBlockNnz, NgsBlocks, gsBlockRowStarts);
+  o_gsBlockRowStarts = platform.malloc<dlong>(NgsBlocks + 1, gsBlockRowStarts);
 }
 
 void ogsOperator_t::SetBlockNnz(int nnz) {
@@ -98,17 +107,18 @@ void ogsOperator_t::ScatterRaw(Type type, Transpose trans, int k, const void *gv
 // is (ogsOperator.cpp:448-479) while its host version honours op
 // (:411-427); here the device honours op as the API documents.
 void ogsOperator_t::GatherScatterRaw(Type type, Op op, Transpose trans, int k, void *v) {
-  if (!NrowBlocksT) return;
-  const dlong *bs = o_blockRowStartsT.ptr();
+  if (!NgsBlocks) return;
+  const dlong nb = NgsBlocks, cap = gsBlockNnz;
+  const dlong *bs = o_gsBlockRowStarts.ptr();
   if (trans == Trans) {
-    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NrowBlocksT, bs, NrowsT, o_rowStartsT.ptr(),
-                                     o_colIdsT.ptr(), NrowsN, o_rowStartsN.ptr(), o_colIdsN.ptr(), blockNnz, v));
+    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, nb, bs, NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(),
+                                     NrowsN, o_rowStartsN.ptr(), o_colIdsN.ptr(), cap, v));
   } else if (trans == Sym) {
-    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NrowBlocksT, bs, NrowsT, o_rowStartsT.ptr(),
-                                     o_colIdsT.ptr(), NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(), blockNnz, v));
+    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, nb, bs, NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(),
+                                     NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(), cap, v));
   } else {
-    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NrowBlocksT, bs, NrowsN, o_rowStartsN.ptr(),
-                                     o_colIdsN.ptr(), NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(), blockNnz, v));
+    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, nb, bs, NrowsN, o_rowStartsN.ptr(), o_colIdsN.ptr(),
+                                     NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(), cap, v));
   }
 }
 

@@ -99,8 +99,11 @@ class Ogs:
                "spxh_ogs_copy_csr")
         return rs, ci[:nnz], bs
 
-    def set_block_nnz(self, cap):
+    def set_block_nnz(self, cap, gs_cap=None):
+        """Row-block LDS tiles: gather/scatter blocks get `cap`, gather+scatter blocks `gs_cap`
+        (default: the same)."""
         checkh(libspxhost().spxh_ogs_set_block_nnz(self.h, cap), "spxh_ogs_set_block_nnz")
+        checkh(libspxhost().spxh_ogs_set_gs_block_nnz(self.h, gs_cap or cap), "spxh_ogs_set_gs_block_nnz")
         self.refresh()
 
     def global_to_local(self):

@@ -117,21 +117,34 @@ def test_all_types_ops_trans_fields(setups, dtype, op, trans, k):
     assert same(dv.to_numpy(), o.scatter(gsrc, base, k, trans))
 
 
-@pytest.mark.parametrize("cap", [64, 512, 1024, 4096])
-def test_result_independent_of_row_blocks(setups, oracle, cap):
+@pytest.mark.parametrize("cap", [64, 512, 1024, 3000, 4096])
+@pytest.mark.parametrize("variant", [0, 1, 3])
+def test_result_independent_of_row_blocks(setups, oracle, cap, variant):
     mesh = (5, 3, 2, 4)
-    say("block cap", mesh, cap)
+    say("block cap", mesh, cap, "variant", variant)
     g, o, _ = setups[mesh]
     q = oracle.uniform(g.N, 11)
     want = o.gather_scatter(q)
+    gq_want = o.gather(q, 1, spx.ADD, spx.TRANS)
     g.set_block_nnz(cap)
+    device.set_tuning("ogs_gs", variant=variant)
     try:
         dq = DeviceArray.from_numpy(q)
         g.gather_scatter(dq)
         device.check_kernel_errors()
         assert same(dq.to_numpy(), want)
+        dq = DeviceArray.from_numpy(q)
+        dgq = DeviceArray(g.NlocalT)
+        g.gather(dgq, dq, 1, spx.ADD, spx.TRANS)
+        assert same(dgq.to_numpy(), gq_want)
+        dout = DeviceArray(g.N)
+        dout.fill_zero()
+        g.scatter(dout, dgq, 1, spx.NOTRANS)
+        device.check_kernel_errors()
+        assert same(dout.to_numpy(), o.scatter(gq_want, np.zeros(g.N), 1, spx.NOTRANS))
     finally:
-        g.set_block_nnz(2048)
+        g.set_block_nnz(2048, 4096)
+        device.set_tuning("ogs_gs", variant=-1)
 
 
 def test_reference_bisection_blocks(setups, oracle):
