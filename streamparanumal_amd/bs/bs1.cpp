@@ -10,11 +10,20 @@ int main(int argc, char **argv) {
     bs::addByteSettings(settings);
     bs::addVerbose(settings);
     bs::addSeed(settings);
+    settings.newSetting("--nontemporal", "--nontemporal", "NONTEMPORAL", "TRUE",
+                        "Streaming (non-temporal) loads and stores", {"TRUE", "FALSE"});
     settings.parseSettings(argc, argv);
     if (settings.helpRequested()) return;
 
     platform_t platform(settings);
     if (settings.compareSetting("VERBOSE", "TRUE")) settings.report();
+    {
+      spx_tuning_t t{};
+      t.xcd_remap = -1;
+      t.variant = -1;
+      t.nontemporal = settings.compareSetting("NONTEMPORAL", "TRUE") ? 1 : 0;
+      SPX_CHECK(spx_set_tuning("bs1", &t));
+    }
     properties_t kernelInfo = platform.props();
     kernelInfo["defines/p_blockSize"] = 256;
     kernel_t kernel = platform.buildKernel("BS/BS1/okl/bs1.okl", "bs1", kernelInfo);

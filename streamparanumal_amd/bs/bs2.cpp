@@ -11,6 +11,8 @@ int main(int argc, char **argv) {
     bs::addByteSettings(settings);
     bs::addVerbose(settings);
     bs::addSeed(settings);
+    settings.newSetting("--nontemporal", "--nontemporal", "NONTEMPORAL", "TRUE",
+                        "Streaming (non-temporal) loads and stores", {"TRUE", "FALSE"});
     settings.newSetting("--alpha", "--alpha", "ALPHA", "1.0", "alpha in y = alpha*x + beta*y");
     settings.newSetting("--beta", "--beta", "BETA", "1.0", "beta in y = alpha*x + beta*y");
     settings.parseSettings(argc, argv);
@@ -18,6 +20,13 @@ int main(int argc, char **argv) {
 
     platform_t platform(settings);
     if (settings.compareSetting("VERBOSE", "TRUE")) settings.report();
+    {
+      spx_tuning_t t{};
+      t.xcd_remap = -1;
+      t.variant = -1;
+      t.nontemporal = settings.compareSetting("NONTEMPORAL", "TRUE") ? 1 : 0;
+      SPX_CHECK(spx_set_tuning("bs2", &t));
+    }
     properties_t kernelInfo = platform.props();
     kernelInfo["defines/p_blockSize"] = 256;
     kernel_t kernel = platform.buildKernel("BS/BS2/okl/bs2.okl", "bs2", kernelInfo);

@@ -41,6 +41,12 @@ class ogsOperator_t {
   memory<dlong> gsBlockRowStarts;
   deviceMemory<dlong> o_gsBlockRowStarts;
 
+  // The non-symmetric gather+scatter stages two value tiles, so its row
+  // blocks hold at most min(blockNnz, 2048) entries.
+  dlong genBlockNnz = defaultBlockNnz;
+  dlong NgenBlocks = 0;
+  deviceMemory<dlong> o_genBlockRowStarts;
+
   Kind kind = Signed;
 
   ogsOperator_t() = default;

@@ -1,5 +1,7 @@
 // ogsOperator_t on the device (libs/ogs/ogsOperator.cpp:176-204, 295-322,
 // 442-481, 597-665) over the libspx ogs kernels.
+#include <algorithm>
+
 #include "ogs.hpp"
 
 namespace libp {
@@ -75,6 +77,15 @@ void ogsOperator_t::SetBlockNnz(int nnz) {
     blockRowStartsN = blockRowStartsT;
     o_blockRowStartsN = o_blockRowStartsT;
   }
+  genBlockNnz = std::min(blockNnz, 2048);
+  if (genBlockNnz == blockNnz) {
+    NgenBlocks = NrowBlocksT;
+    o_genBlockRowStarts = o_blockRowStartsT;
+  } else {
+    memory<dlong> starts;
+    blockRowsGreedy(NrowsT, rowStartsT, genBlockNnz, NgenBlocks, starts);
+    o_genBlockRowStarts = platform.malloc<dlong>(NgenBlocks + 1, starts);
+  }
 }
 
 void ogsOperator_t::Free() { *this = ogsOperator_t(); }
@@ -107,18 +118,21 @@ void ogsOperator_t::ScatterRaw(Type type, Transpose trans, int k, const void *gv
 // is (ogsOperator.cpp:448-479) while its host version honours op
 // (:411-427); here the device honours op as the API documents.
 void ogsOperator_t::GatherScatterRaw(Type type, Op op, Transpose trans, int k, void *v) {
-  if (!NgsBlocks) return;
-  const dlong nb = NgsBlocks, cap = gsBlockNnz;
-  const dlong *bs = o_gsBlockRowStarts.ptr();
+  if (trans == Sym) {
+    if (NgsBlocks)
+      SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NgsBlocks, o_gsBlockRowStarts.ptr(), NrowsT,
+                                       o_rowStartsT.ptr(), o_colIdsT.ptr(), NrowsT, o_rowStartsT.ptr(),
+                                       o_colIdsT.ptr(), gsBlockNnz, v));
+    return;
+  }
+  if (!NgenBlocks) return;
+  const dlong *bs = o_genBlockRowStarts.ptr();
   if (trans == Trans) {
-    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, nb, bs, NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(),
-                                     NrowsN, o_rowStartsN.ptr(), o_colIdsN.ptr(), cap, v));
-  } else if (trans == Sym) {
-    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, nb, bs, NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(),
-                                     NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(), cap, v));
+    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NgenBlocks, bs, NrowsT, o_rowStartsT.ptr(),
+                                     o_colIdsT.ptr(), NrowsN, o_rowStartsN.ptr(), o_colIdsN.ptr(), genBlockNnz, v));
   } else {
-    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, nb, bs, NrowsN, o_rowStartsN.ptr(), o_colIdsN.ptr(),
-                                     NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(), cap, v));
+    SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NgenBlocks, bs, NrowsN, o_rowStartsN.ptr(),
+                                     o_colIdsN.ptr(), NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(), genBlockNnz, v));
   }
 }
 
