@@ -18,6 +18,7 @@ the oracle's restatement of the reference's Serial mode on a bounded sample.
 """
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -139,7 +140,35 @@ def main():
     h, e = timed(lambda: g.scatter(q, gq, 1, spx.NOTRANS), K, W)
     entry("BS7_scatter", bytes_bs7, Ng, h, e)
     device.check_kernel_errors()
-    del q, gq, g
+
+    # ------------------------------------- BS8 on FP32 / BF16 storage (FP64 sums)
+    red = streaming.Reducer(Nl, spx.REDUCE_FAST, 256)
+
+    def rel_err(approx64, exact64):
+        nref = (red.norm(exact64), red.value())[1]
+        streaming.axpy(1.0, exact64, -1.0, approx64)
+        return math.sqrt((red.norm(approx64), red.value())[1] / nref)
+
+    device.fill_uniform(q, 7, 0)
+    q64 = DeviceArray(Nl)
+    for st, dt, tag, sz in ((spx.STORE_F32, np.float32, "f32", 4), (spx.STORE_BF16, np.uint16, "bf16", 2)):
+        qs = DeviceArray(Nl, dt)
+        device.convert(q, qs)
+        h, e = timed(lambda: g.gather_scatter_mixed(qs), K, W)
+        device.check_kernel_errors()
+        # error of one application against the FP64 path (bit-exact with the oracle)
+        device.convert(q, qs)
+        g.gather_scatter_mixed(qs)
+        device.convert(qs, q64)
+        ref = DeviceArray(Nl)
+        streaming.copy(q, ref)
+        g.gather_scatter(ref)
+        err = rel_err(q64, ref)
+        bytes_m = 4 * (Ng + 1) + 4 * Nl + 2 * sz * Nl
+        entry(f"BS8_gather_scatter_{tag}", bytes_m, Ng, h, e,
+              {"rel_err_vs_f64": float(f"{err:.3e}"), "speedup_vs_f64": round(ev8 / e, 3)})
+        del qs, ref
+    del q, gq, g, q64, red
 
     # ---------------------------------------------------------------- BS1-BS5
     n = 1 << args.log2n
@@ -165,7 +194,37 @@ def main():
     red = streaming.Reducer(n, spx.REDUCE_FAST, 256)
     h, e = timed(lambda: red.cg_full(1e-9, pvec, a, xvec, b), K, W)
     entry("BS5_cg_full_fast", 48 * n, n, h, e)
-    del pvec, xvec, a, b
+    red_ref = streaming.Reducer(n, spx.REDUCE_REFERENCE, 256)
+    h, e64 = timed(lambda: red_ref.cg_full(1e-9, pvec, a, xvec, b), K, W)
+    entry("BS5_cg_full_reference", 48 * n, n, h, e64)
+    # CG update on FP32 / BF16 storage, FP64 rdotr; error of one update vs the FP64 path
+    alpha = 0.31
+    mix = streaming.MixedCG()
+    r64 = DeviceArray(n)
+    for st, dt, tag, sz in ((spx.STORE_F32, np.float32, "f32", 4), (spx.STORE_BF16, np.uint16, "bf16", 2)):
+        vs = [DeviceArray(n, dt) for _ in range(4)]
+        for v, src in zip(vs, (pvec, a, xvec, b)):
+            device.convert(src, v)
+        h, e = timed(lambda: mix.cg_full(1e-9, *vs), K, W)
+        for v, src in zip(vs, (pvec, a, xvec, b)):
+            device.convert(src, v)
+        mix.cg_full(alpha, *vs)
+        rd_mixed = mix.value()
+        x0, r0 = DeviceArray(n), DeviceArray(n)
+        streaming.copy(xvec, x0)
+        streaming.copy(b, r0)
+        red_ref.cg_full(alpha, pvec, a, x0, r0)
+        rd64 = red_ref.value()
+        device.convert(vs[3], r64)
+        nref = (red_ref.norm(r0), red_ref.value())[1]
+        streaming.axpy(1.0, r0, -1.0, r64)
+        err_r = math.sqrt((red_ref.norm(r64), red_ref.value())[1] / nref)
+        entry(f"BS5_cg_full_{tag}", 6 * sz * n, n, h, e,
+              {"rel_err_r_vs_f64": float(f"{err_r:.3e}"),
+               "rel_err_rdotr_vs_f64": float(f"{abs(rd_mixed - rd64) / rd64:.3e}"),
+               "speedup_vs_f64_reference": round(e64 / e, 3)})
+        del vs, x0, r0
+    del pvec, xvec, a, b, r64
     for name, d in suite.items():
         d["pct_measured_peak"] = round(100 * d["GBs"] / measured_peak, 1)
 

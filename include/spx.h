@@ -191,8 +191,36 @@ int spx_ogs_gather_scatter(int type, int op, int K, int32_t nblocks, const int32
                            int32_t ngrows, const int32_t *gStarts, const int32_t *gIds,
                            int32_t nsrows, const int32_t *sStarts, const int32_t *sIds,
                            int32_t max_block_nnz, void *q);
+/* scatter through the local->gathered map g2l (n entries, -1 = in no row):
+   v[k + l*K] = gv[k + g2l[l]*K].  Same result as spx_ogs_scatter over a CSR
+   whose rows partition the local nodes (every ogs operator's do); the map is
+   the reference's GlobalToLocal (ogsSetup.cpp:833-849).  gv must hold at
+   least one row (K values) even if every map entry is -1. */
+int spx_ogs_scatter_inverse(int type, int K, int32_t n, const int32_t *g2l, const void *gv, void *v);
 /* extract (ogsKernels.okl:167-177): gq[k + n*K] = q[k + ids[n]*K], n < n_ids */
 int spx_ogs_extract(int type, int K, int32_t n_ids, const int32_t *ids, const void *q, void *gq);
+
+/* ------------------------------------------- reduced-precision storage -- */
+/* Storage types of the mixed-precision variants (no reference counterpart:
+   the reference's dfloat is one type for storage and arithmetic,
+   include/types.h:36-44 and README.md:49-53). */
+enum { SPX_STORE_F64 = 0, SPX_STORE_F32 = 1, SPX_STORE_BF16 = 2 };
+/* partials needed by spx_bs5_full_mixed's tmp */
+enum { SPX_MIXED_TMP_COUNT = 256 };
+
+/* ogs::Sym Add gatherScatter (ogsKernels.okl:32-81) on a vector stored as
+   `storage`: each row's sum is taken in FP64 over its members in stored
+   order and rounded once to the storage type.  SPX_STORE_F64 runs
+   spx_ogs_gather_scatter.  Arguments as spx_ogs_gather_scatter's Sym case. */
+int spx_ogs_gather_scatter_mixed(int storage, int K, int32_t nblocks, const int32_t *blockStarts,
+                                 const int32_t *rowStarts, const int32_t *colIds,
+                                 int32_t max_block_nnz, void *q);
+/* CG update (README.md:49-53) on FP32 or BF16 vectors: x += alpha*p and
+   r -= alpha*Ap in FP32 (alpha rounded to FP32), results rounded to the
+   storage type; *rdotr = sum of the stored r squared, accumulated in FP64.
+   Vectors aligned to 4 entries; tmp holds SPX_MIXED_TMP_COUNT doubles. */
+int spx_bs5_full_mixed(int storage, int64_t n, double alpha, const void *p, const void *Ap, void *x,
+                       void *r, double *tmp, double *rdotr);
 
 /* Kernel launch parameters chosen by the autotuner (0 = library default). */
 typedef struct {

@@ -55,6 +55,9 @@ int spxh_ogs_copy_csr(spxh_ogs_t ogs, int which, int32_t *rowStarts, int32_t *co
    blocks, and gather+scatter blocks */
 int spxh_ogs_set_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
 int spxh_ogs_set_gs_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
+/* scatter path: 1 = over the CSR row blocks (ogsKernels.okl scatter; the
+   default), 0 = through the local->row map in local order */
+int spxh_ogs_set_scatter_path(spxh_ogs_t ogs, int by_rows);
 /* ogs_t::SetupGlobalToLocalMapping (ogsSetup.cpp:833-849) */
 int spxh_ogs_global_to_local(spxh_ogs_t ogs, int32_t *g2l);
 
@@ -63,6 +66,10 @@ int spxh_ogs_global_to_local(spxh_ogs_t ogs, int32_t *g2l);
 int spxh_ogs_gather_scatter(spxh_ogs_t ogs, int type, int op, int trans, int k, void *v);
 int spxh_ogs_gather(spxh_ogs_t ogs, int type, int op, int trans, int k, const void *v, void *gv);
 int spxh_ogs_scatter(spxh_ogs_t ogs, int type, int trans, int k, const void *gv, void *v);
+/* Sym Add gatherScatter of v stored as SPX_STORE_F32 / SPX_STORE_BF16 (FP64
+   row sums, one rounding; spx_ogs_gather_scatter_mixed); no reference
+   counterpart, the reference gathers dfloat only (ogs.cpp:39-125) */
+int spxh_ogs_gather_scatter_mixed(spxh_ogs_t ogs, int storage, int k, void *v);
 
 /* The host half of the setup alone (no device needed): grouping, unique
    flagging (written back into ids), row numbering, T- and N-CSR. */

@@ -40,6 +40,11 @@ _FUNCS = {
     "oracle_ogs_gather_scatter": (None, [C.POINTER(OgsStruct), C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "oracle_csr_gs": (None, [i32, vp, vp, C.c_int, vp]),
     "oracle_ogs_gs_okl": (None, [i32, C.c_int, vp, vp, vp, vp, vp, vp]),
+    "oracle_bf16_from_double": (C.c_uint16, [dbl]),
+    "oracle_bf16_from_float": (C.c_uint16, [C.c_float]),
+    "oracle_bf16_to_double": (dbl, [C.c_uint16]),
+    "oracle_gs_mixed": (None, [C.c_int, i32, vp, vp, C.c_int, vp]),
+    "oracle_cg_mixed": (dbl, [C.c_int, i64, dbl, vp, vp, vp, vp]),
     "oracle_time_bs8": (dbl, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(dbl),
                               C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "oracle_time_stream": (dbl, [C.c_int, i64, C.c_int, C.c_int]),
@@ -208,6 +213,53 @@ def csr_gs(rowStarts, colIds, q, k=1):
     ci = np.ascontiguousarray(colIds, dtype=np.int32)
     lib().oracle_csr_gs(rs.size - 1, _p(rs), _p(ci), k, _p(q))
     return q
+
+
+STORE_F64, STORE_F32, STORE_BF16 = 0, 1, 2
+
+
+def to_storage(v, storage):
+    """Round float64 values to the storage type (float32 array, or bf16 bits as uint16)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    if storage == STORE_F32:
+        return v.astype(np.float32)
+    if storage == STORE_BF16:
+        f = lib().oracle_bf16_from_double
+        return np.array([f(float(x)) for x in v], dtype=np.uint16) if v.size < 4096 else _bf16_vec(v)
+    return v.copy()
+
+
+def _bf16_vec(v):
+    # exact nearest-even rounding, vectorised (same rule as oracle_bf16_from_double)
+    f = v.astype(np.float32)
+    lo = f.view(np.uint32) & np.uint32(0xFFFF0000)
+    up = np.abs(lo.view(np.float32).astype(np.float64)) > np.abs(v)
+    lo = np.where(up, lo - np.uint32(0x10000), lo).astype(np.uint32)
+    hi = (lo + np.uint32(0x10000)).astype(np.uint32)
+    dl = np.abs(v - lo.view(np.float32).astype(np.float64))
+    dh = np.abs(hi.view(np.float32).astype(np.float64) - v)
+    r = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where((lo >> 16) & 1, hi, lo)))
+    return (r >> 16).astype(np.uint16)
+
+
+def from_storage(q, storage):
+    if storage == STORE_BF16:
+        return (q.astype(np.uint32) << np.uint32(16)).view(np.float32).astype(np.float64)
+    return np.asarray(q, dtype=np.float64)
+
+
+def gs_mixed(storage, rowStarts, colIds, q, k=1):
+    q = np.array(q, copy=True)
+    rs = np.ascontiguousarray(rowStarts, dtype=np.int32)
+    ci = np.ascontiguousarray(colIds, dtype=np.int32)
+    lib().oracle_gs_mixed(storage, rs.size - 1, _p(rs), _p(ci), k, _p(q))
+    return q
+
+
+def cg_mixed(storage, alpha, p, Ap, x, r):
+    x, r = np.array(x, copy=True), np.array(r, copy=True)
+    v = lib().oracle_cg_mixed(storage, p.size, alpha, _p(p), _p(Ap), _p(x), _p(r))
+    return v, x, r
 
 
 def time_bs8(nx, p, reps, threads):

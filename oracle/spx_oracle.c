@@ -714,6 +714,87 @@ void oracle_ogs_gs_okl(int32_t nblocks, int K, const int32_t *blockStarts,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Reduced-precision storage                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* nearest-even bf16 of a double, decided exactly: candidates are the bf16
+   values just below and above x in magnitude (bf16 values are doubles) */
+uint16_t oracle_bf16_from_double(double x) {
+  if (x != x) return 0x7fc0;
+  float f = (float)x;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  uint32_t lo = u & 0xffff0000u;
+  float flo;
+  memcpy(&flo, &lo, 4);
+  if (fabs((double)flo) > fabs(x)) lo -= 0x10000u; /* f rounded up past x in magnitude */
+  uint32_t hi = lo + 0x10000u;
+  float fl, fh;
+  memcpy(&fl, &lo, 4);
+  memcpy(&fh, &hi, 4);
+  const double dl = fabs(x - (double)fl), dh = fabs((double)fh - x);
+  uint32_t r;
+  if (dl < dh) r = lo;
+  else if (dh < dl) r = hi;
+  else r = ((lo >> 16) & 1u) ? hi : lo;
+  return (uint16_t)(r >> 16);
+}
+
+uint16_t oracle_bf16_from_float(float x) { return oracle_bf16_from_double((double)x); }
+
+double oracle_bf16_to_double(uint16_t b) {
+  uint32_t u = ((uint32_t)b) << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+static double load_s(int storage, const void *q, int64_t i) {
+  if (storage == ORACLE_STORE_F32) return (double)((const float *)q)[i];
+  if (storage == ORACLE_STORE_BF16) return oracle_bf16_to_double(((const uint16_t *)q)[i]);
+  return ((const double *)q)[i];
+}
+
+static void store_s(int storage, void *q, int64_t i, double v) {
+  if (storage == ORACLE_STORE_F32) ((float *)q)[i] = (float)v;
+  else if (storage == ORACLE_STORE_BF16) ((uint16_t *)q)[i] = oracle_bf16_from_double(v);
+  else ((double *)q)[i] = v;
+}
+
+void oracle_gs_mixed(int storage, int32_t nrows, const int32_t *rs, const int32_t *ci, int K, void *q) {
+  for (int32_t n = 0; n < nrows; ++n)
+    for (int k = 0; k < K; ++k) {
+      double acc = 0.0;
+      for (int32_t g = rs[n]; g < rs[n + 1]; ++g) acc += load_s(storage, q, k + (int64_t)ci[g] * K);
+      for (int32_t g = rs[n]; g < rs[n + 1]; ++g) store_s(storage, q, k + (int64_t)ci[g] * K, acc);
+    }
+}
+
+static float load_f(int storage, const void *q, int64_t i) { return (float)load_s(storage, q, i); }
+
+static void store_f(int storage, void *q, int64_t i, float v) {
+  if (storage == ORACLE_STORE_BF16) ((uint16_t *)q)[i] = oracle_bf16_from_float(v);
+  else if (storage == ORACLE_STORE_F32) ((float *)q)[i] = v;
+  else ((double *)q)[i] = (double)v;
+}
+
+double oracle_cg_mixed(int storage, int64_t n, double alpha, const void *p, const void *Ap, void *x, void *r) {
+  const float a = (float)alpha;
+  long double acc = 0.0L;
+  for (int64_t i = 0; i < n; ++i) {
+    const float ap = a * load_f(storage, p, i);
+    const float xn = load_f(storage, x, i) + ap;
+    const float aAp = a * load_f(storage, Ap, i);
+    const float rn = load_f(storage, r, i) - aAp;
+    store_f(storage, x, i, xn);
+    store_f(storage, r, i, rn);
+    const long double rs = load_s(storage, r, i);
+    acc += rs * rs;
+  }
+  return (double)acc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* CPU baseline timers                                                       */
 /* ------------------------------------------------------------------------ */
 

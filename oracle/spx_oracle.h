@@ -84,6 +84,20 @@ void oracle_ogs_gs_okl(int32_t nblocks, int K, const int32_t *blockStarts,
                        const int32_t *gStarts, const int32_t *gIds,
                        const int32_t *sStarts, const int32_t *sIds, double *q);
 
+/* ---- reduced-precision storage variants (no reference kernel; the
+   semantics are this build's, restated here for bit-exact checks) ---- */
+enum { ORACLE_STORE_F64 = 0, ORACLE_STORE_F32 = 1, ORACLE_STORE_BF16 = 2 };
+uint16_t oracle_bf16_from_double(double x);     /* round to nearest even */
+uint16_t oracle_bf16_from_float(float x);       /* round to nearest even */
+double oracle_bf16_to_double(uint16_t b);
+/* Sym Add gather+scatter of values stored as `storage`, each row summed in
+   FP64 in stored order and rounded once to the storage type */
+void oracle_gs_mixed(int storage, int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int K,
+                     void *q);
+/* x += alpha p, r -= alpha Ap in FP32 arithmetic on values stored as
+   `storage`; returns sum over the stored r of r^2 in long double */
+double oracle_cg_mixed(int storage, int64_t n, double alpha, const void *p, const void *Ap, void *x, void *r);
+
 /* ---- CPU baseline timers (Serial = 1 thread; OpenMP = all host cores) ---- */
 double oracle_time_bs8(int nx, int p, int reps, int threads, double *checksum,
                        double *setup_seconds, int64_t *nlocal, int64_t *nglobal);

@@ -8,8 +8,9 @@ Python view of the same libraries.
 """
 from . import _lib
 from ._lib import (ADD, DOUBLE, FLOAT, HALO, INT32, INT64, MAX, MIN, MUL, NOTRANS, REDUCE_FAST,
-                   REDUCE_REFERENCE, SIGNED, SYM, TRANS, UNSIGNED, SpxError, libspx, libspxhost)
+                   REDUCE_REFERENCE, SIGNED, STORE_BF16, STORE_F32, STORE_F64, SYM, TRANS, UNSIGNED, SpxError,
+                   libspx, libspxhost)
 
 __all__ = ["_lib", "libspx", "libspxhost", "SpxError", "ADD", "MUL", "MAX", "MIN", "SYM", "NOTRANS",
            "TRANS", "FLOAT", "DOUBLE", "INT32", "INT64", "UNSIGNED", "SIGNED", "HALO", "REDUCE_REFERENCE",
-           "REDUCE_FAST"]
+           "REDUCE_FAST", "STORE_F64", "STORE_F32", "STORE_BF16"]

@@ -16,6 +16,23 @@ ADD, MUL, MAX, MIN = 0, 1, 2, 3
 SYM, NOTRANS, TRANS = 0, 1, 2
 UNSIGNED, SIGNED, HALO = 0, 1, 2
 REDUCE_REFERENCE, REDUCE_FAST = 0, 1
+# storage types of the reduced-precision variants; BF16 vectors are held as
+# their raw 16 bits (numpy uint16)
+STORE_F64, STORE_F32, STORE_BF16 = 0, 1, 2
+MIXED_TMP_COUNT = 256
+
+
+def storage_of(dtype):
+    """SPX_STORE_* for a numpy dtype: float64, float32, or uint16 (bf16 bits)."""
+    import numpy as _np
+    d = _np.dtype(dtype)
+    if d == _np.float64:
+        return STORE_F64
+    if d == _np.float32:
+        return STORE_F32
+    if d == _np.uint16:
+        return STORE_BF16
+    raise TypeError(f"no storage type for {d}")
 
 vp, i32, i64, u64, dbl, cstr, u8 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_char_p, C.c_uint8
 pi32, pi64, pint, pflt = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_float)
@@ -88,7 +105,10 @@ SPX_FUNCS = {
     "spx_ogs_gather": (C.c_int, [C.c_int, C.c_int, C.c_int, i32, vp, vp, vp, i32, vp, vp]),
     "spx_ogs_scatter": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, i32, vp, vp]),
     "spx_ogs_gather_scatter": (C.c_int, [C.c_int, C.c_int, C.c_int, i32, vp, i32, vp, vp, i32, vp, vp, i32, vp]),
+    "spx_ogs_scatter_inverse": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp]),
     "spx_ogs_extract": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp]),
+    "spx_ogs_gather_scatter_mixed": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, i32, vp]),
+    "spx_bs5_full_mixed": (C.c_int, [C.c_int, C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp]),
     "spx_set_tuning": (C.c_int, [cstr, C.POINTER(Tuning)]),
     "spx_get_tuning": (C.c_int, [cstr, C.POINTER(Tuning)]),
 }
@@ -103,8 +123,10 @@ SPXH_FUNCS = {
     "spxh_ogs_copy_csr": (C.c_int, [vp, C.c_int, vp, vp, vp]),
     "spxh_ogs_set_block_nnz": (C.c_int, [vp, i32]),
     "spxh_ogs_set_gs_block_nnz": (C.c_int, [vp, i32]),
+    "spxh_ogs_set_scatter_path": (C.c_int, [vp, C.c_int]),
     "spxh_ogs_global_to_local": (C.c_int, [vp, vp]),
     "spxh_ogs_gather_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+    "spxh_ogs_gather_scatter_mixed": (C.c_int, [vp, C.c_int, C.c_int, vp]),
     "spxh_ogs_gather": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_ogs_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_csr_build": (C.c_int, [i32, vp, C.c_int, C.c_int, C.POINTER(vp)]),

@@ -148,6 +148,22 @@ int spx_ogs_gather_scatter(int type, int op, int K, int32_t nblocks, const int32
   return SPX_OK;
 }
 
+int spx_ogs_scatter_inverse(int type, int K, int32_t n, const int32_t *g2l, const void *gv, void *v) {
+  SPX_NEED_DEVICE();
+  if (!type_size(type)) return fail(SPX_ERR_ARG, "spx_ogs_scatter_inverse: unknown type");
+  if (K < 1 || K > 65535 || n < 0) return fail(SPX_ERR_ARG, "spx_ogs_scatter_inverse: bad K or n");
+  if (n == 0) return SPX_OK;
+  if (!g2l || !gv || !v) return fail(SPX_ERR_ARG, "spx_ogs_scatter_inverse: null pointer");
+  int rc = by_type(type, [&](auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    spx_tuning_t t = tuning_for("ogs_scatter_inverse", spx_tuning_t{});
+    return launch_scatter_inv(current_stream(), (int64_t)n, K, g2l, (const T *)gv, (T *)v, t.unroll > 0 ? t.unroll : 8);
+  });
+  if (rc) return rc;
+  SPX_LAUNCHED("spx_ogs_scatter_inverse");
+  return SPX_OK;
+}
+
 int spx_ogs_extract(int type, int K, int32_t n_ids, const int32_t *ids, const void *q, void *gq) {
   SPX_NEED_DEVICE();
   if (!type_size(type)) return fail(SPX_ERR_ARG, "spx_ogs_extract: unknown type");

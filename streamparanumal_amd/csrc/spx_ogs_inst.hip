@@ -81,6 +81,28 @@ int launch_scatter(const Launch &L, const int *bs, const int *st, const int *ids
   });
 }
 
+int launch_scatter_inv(hipStream_t s, int64_t n, int K, const int *g2l, const T *gv, T *v, int npl) {
+  npl = npl >= 16 ? 16 : (npl >= 8 ? 8 : 4);
+  const int64_t per_block = (int64_t)npl * kThreads;
+  const int64_t blocks = (n + per_block - 1) / per_block;
+  if (blocks > INT32_MAX) return fail(SPX_ERR_ARG, "spx_ogs_scatter_inverse: n too large");
+  const int kf = (K >= 1 && K <= 4) ? K : 1;
+  const dim3 grid((unsigned)blocks, kf == 1 ? K : 1);
+  auto go = [&](auto npl_c) {
+    constexpr int NPL = decltype(npl_c)::value;
+    switch (kf) {
+      case 1: hipLaunchKernelGGL((k_scatter_inv<T, 1, NPL>), grid, dim3(kThreads), 0, s, n, K, g2l, gv, v); break;
+      case 2: hipLaunchKernelGGL((k_scatter_inv<T, 2, NPL>), grid, dim3(kThreads), 0, s, n, K, g2l, gv, v); break;
+      case 3: hipLaunchKernelGGL((k_scatter_inv<T, 3, NPL>), grid, dim3(kThreads), 0, s, n, K, g2l, gv, v); break;
+      default: hipLaunchKernelGGL((k_scatter_inv<T, 4, NPL>), grid, dim3(kThreads), 0, s, n, K, g2l, gv, v); break;
+    }
+  };
+  if (npl == 16) go(std::integral_constant<int, 16>{});
+  else if (npl == 8) go(std::integral_constant<int, 8>{});
+  else go(std::integral_constant<int, 4>{});
+  return SPX_OK;
+}
+
 int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq) {
   int64_t g = (total + kThreads - 1) / kThreads;
   if (g > 65536) g = 65536;

@@ -378,6 +378,40 @@ __global__ __launch_bounds__(kThreads) void k_scatter(Launch L, const int *__res
       for (int f = 0; f < KF; ++f) v[(int64_t)cid[j] * K + kb + f] = vals[(t + j * kThreads) * KF + f];
 }
 
+// ------------------------------------------------------- inverse scatter --
+// v[k + l*K] = gv[k + g2l[l]*K] for every local node l with g2l[l] >= 0: the
+// same result as the CSR scatter (each local node sits in at most one row),
+// with reads of g2l and writes of v in local order.  Each lane handles NPL
+// consecutive nodes (NPL/4 16-byte map loads, then NPL row-value loads).
+template <typename T, int KF, int NPL>
+__global__ __launch_bounds__(kThreads) void k_scatter_inv(int64_t n, int K, const int *__restrict__ g2l,
+                                                          const T *__restrict__ gv, T *__restrict__ v) {
+  const int kb = blockIdx.y * KF;
+  const int64_t l0 = ((int64_t)blockIdx.x * kThreads + threadIdx.x) * NPL;
+  if (l0 >= n) return;
+  int r[NPL];
+  if (l0 + NPL <= n) {
+#pragma unroll
+    for (int u = 0; u < NPL; u += 4) {
+      const int4 x = *reinterpret_cast<const int4 *>(g2l + l0 + u);
+      r[u] = x.x; r[u + 1] = x.y; r[u + 2] = x.z; r[u + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < NPL; ++u) r[u] = (l0 + u < n) ? g2l[l0 + u] : -1;
+  }
+  T x[NPL][KF];
+#pragma unroll
+  for (int u = 0; u < NPL; ++u)
+#pragma unroll
+    for (int f = 0; f < KF; ++f) x[u][f] = gv[(int64_t)(r[u] < 0 ? 0 : r[u]) * K + kb + f];
+#pragma unroll
+  for (int u = 0; u < NPL; ++u)
+    if (r[u] >= 0)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) v[(l0 + u) * K + kb + f] = x[u][f];
+}
+
 // --------------------------------------------------------------- extract --
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_extract(int64_t total, int K, const int *__restrict__ ids,
@@ -422,7 +456,8 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
                     const int *gi, const int *ss, const int *si, T *q);                                    \
   int launch_gather(const Launch &L, int op, const int *bs, const int *st, const int *ids, const T *v, T *gv); \
   int launch_scatter(const Launch &L, const int *bs, const int *st, const int *ids, const T *gv, T *v);    \
-  int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq);
+  int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq);           \
+  int launch_scatter_inv(hipStream_t s, int64_t n, int K, const int *g2l, const T *gv, T *v, int npl);
 SPX_OGS_GEN_DECL(float)
 SPX_OGS_GEN_DECL(double)
 SPX_OGS_GEN_DECL(int)

@@ -165,3 +165,19 @@ def set_tuning(kernel, block_threads=0, xcd_remap=-1, unroll=0, grid_blocks=0, n
 def fill_uniform(arr, seed, skip=0):
     """U[-1,1) from the reference LCG (random.cpp:45-163), element i = draw skip+i."""
     check(libspx().spx_fill_uniform_f64(arr.n, arr.ptr, seed, skip), "spx_fill_uniform_f64")
+
+
+def convert(src, dst, n=None):
+    """Convert between FP64 and a storage type on the device: float64 -> float32 / uint16 (bf16,
+    nearest even), or float32 / uint16 -> float64 (exact)."""
+    n = src.n if n is None else n
+    if n > min(src.n, dst.n):
+        raise ValueError("convert: n exceeds an array")
+    pair = (src.dtype, dst.dtype)
+    fns = {(np.dtype(np.float64), np.dtype(np.float32)): "spx_convert_f64_to_f32",
+           (np.dtype(np.float64), np.dtype(np.uint16)): "spx_convert_f64_to_bf16",
+           (np.dtype(np.float32), np.dtype(np.float64)): "spx_convert_f32_to_f64",
+           (np.dtype(np.uint16), np.dtype(np.float64)): "spx_convert_bf16_to_f64"}
+    if pair not in fns:
+        raise TypeError(f"convert: no conversion {pair[0]} -> {pair[1]}")
+    check(getattr(libspx(), fns[pair])(n, src.ptr, dst.ptr), fns[pair])

@@ -41,6 +41,12 @@ class ogsOperator_t {
   memory<dlong> gsBlockRowStarts;
   deviceMemory<dlong> o_gsBlockRowStarts;
 
+  // Local -> gathered row maps of the T- and N-CSR (-1: in no row), for the
+  // scatter in local order (spx_ogs_scatter_inverse).  Measured on MI355X at
+  // 64^3, p=7 the CSR-order scatter is faster, so it is the default.
+  deviceMemory<dlong> o_localRowT, o_localRowN;
+  bool scatterByRows = true;  // false: scatter through the local -> row maps
+
   // The non-symmetric gather+scatter stages two value tiles, so its row
   // blocks hold at most min(blockNnz, 2048) entries.
   dlong genBlockNnz = defaultBlockNnz;
@@ -70,6 +76,9 @@ class ogsOperator_t {
   void GatherRaw(Type type, Op op, Transpose trans, int k, const void *v, void *gv);
   void ScatterRaw(Type type, Transpose trans, int k, const void *gv, void *v);
   void GatherScatterRaw(Type type, Op op, Transpose trans, int k, void *v);
+  // Sym Add gatherScatter on FP32 / BF16 storage with FP64 row sums
+  // (storage: SPX_STORE_*)
+  void GatherScatterMixedRaw(int storage, int k, void *v);
 };
 
 // The local operator's CSR pair as built by the setup (host arrays).

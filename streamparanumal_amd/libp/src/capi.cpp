@@ -148,6 +148,15 @@ int spxh_ogs_set_gs_block_nnz(spxh_ogs_t h, int32_t nnz) {
   });
 }
 
+int spxh_ogs_set_scatter_path(spxh_ogs_t h, int by_rows) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_scatter_path: null");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    op->scatterByRows = by_rows != 0;
+  });
+}
+
 int spxh_ogs_global_to_local(spxh_ogs_t h, int32_t *g2l) {
   if (!h || !g2l) return err(SPX_ERR_ARG, "spxh_ogs_global_to_local: null");
   return guarded([&] {
@@ -172,6 +181,16 @@ int spxh_ogs_gather_scatter(spxh_ogs_t h, int type, int op, int trans, int k, vo
     auto o = h->ogs.localOperator();
     LIBP_ABORT("ogs not set up", !o);
     o->GatherScatterRaw((ogs::Type)type, (ogs::Op)op, (ogs::Transpose)trans, k, v);
+  });
+}
+
+int spxh_ogs_gather_scatter_mixed(spxh_ogs_t h, int storage, int k, void *v) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_gather_scatter_mixed: null handle");
+  if (k < 1) return err(SPX_ERR_ARG, "k must be >= 1");
+  return guarded([&] {
+    auto o = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !o);
+    o->GatherScatterMixedRaw(storage, k, v);
   });
 }
 

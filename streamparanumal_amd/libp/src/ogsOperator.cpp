@@ -55,6 +55,15 @@ ogsOperator_t::ogsOperator_t(platform_t &platform_, Kind kind_, const dlong Nrow
   }
   SetBlockNnz(blockNnz_);
   SetGsBlockNnz(gsBlockNnz_);
+  auto localRows = [&](dlong nrows, const memory<dlong> &rs, const memory<dlong> &ci) {
+    memory<dlong> m(Ncols > 0 ? Ncols : 1);
+    std::fill(m.ptr(), m.ptr() + Ncols, -1);
+    for (dlong r = 0; r < nrows; ++r)
+      for (dlong i = rs[r]; i < rs[r + 1]; ++i) m[ci[i]] = r;
+    return platform.malloc<dlong>(Ncols, m);
+  };
+  o_localRowT = localRows(NrowsT, rowStartsT, colIdsT);
+  o_localRowN = (kind == Signed) ? localRows(NrowsN, rowStartsN, colIdsN) : o_localRowT;
 }
 
 void ogsOperator_t::SetGsBlockNnz(int nnz) {
@@ -103,6 +112,12 @@ void ogsOperator_t::GatherRaw(Type type, Op op, Transpose trans, int k, const vo
 }
 
 void ogsOperator_t::ScatterRaw(Type type, Transpose trans, int k, const void *gv, void *v) {
+  if (!scatterByRows) {
+    const dlong nrows = (trans == Trans) ? NrowsN : NrowsT;
+    if (nrows && Ncols)
+      SPX_CHECK(spx_ogs_scatter_inverse((int)type, k, Ncols, (trans == Trans ? o_localRowN : o_localRowT).ptr(), gv, v));
+    return;
+  }
   if (trans == Trans) {
     if (NrowBlocksN)
       SPX_CHECK(spx_ogs_scatter((int)type, k, NrowBlocksN, o_blockRowStartsN.ptr(), o_rowStartsN.ptr(),
@@ -134,6 +149,12 @@ void ogsOperator_t::GatherScatterRaw(Type type, Op op, Transpose trans, int k, v
     SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NgenBlocks, bs, NrowsN, o_rowStartsN.ptr(),
                                      o_colIdsN.ptr(), NrowsT, o_rowStartsT.ptr(), o_colIdsT.ptr(), genBlockNnz, v));
   }
+}
+
+void ogsOperator_t::GatherScatterMixedRaw(int storage, int k, void *v) {
+  if (NgsBlocks)
+    SPX_CHECK(spx_ogs_gather_scatter_mixed(storage, k, NgsBlocks, o_gsBlockRowStarts.ptr(), o_rowStartsT.ptr(),
+                                           o_colIdsT.ptr(), gsBlockNnz, v));
 }
 
 template <typename T>

@@ -106,6 +106,10 @@ class Ogs:
         checkh(libspxhost().spxh_ogs_set_gs_block_nnz(self.h, gs_cap or cap), "spxh_ogs_set_gs_block_nnz")
         self.refresh()
 
+    def set_scatter_path(self, by_rows):
+        """Scatter over the CSR row blocks (True, default) or through the local->row map (False)."""
+        checkh(libspxhost().spxh_ogs_set_scatter_path(self.h, 1 if by_rows else 0), "spxh_ogs_set_scatter_path")
+
     def global_to_local(self):
         g = np.empty(self.N, dtype=np.int32)
         checkh(libspxhost().spxh_ogs_global_to_local(self.h, g.ctypes.data), "spxh_ogs_global_to_local")
@@ -114,6 +118,11 @@ class Ogs:
     def gather_scatter(self, q, k=1, op=_lib.ADD, trans=_lib.SYM):
         checkh(libspxhost().spxh_ogs_gather_scatter(self.h, ogs_type(q.dtype), op, trans, k, q.ptr),
                "spxh_ogs_gather_scatter")
+
+    def gather_scatter_mixed(self, q, k=1):
+        """Sym Add gatherScatter of an FP32 or BF16 (uint16 bits) vector, FP64 row sums."""
+        checkh(libspxhost().spxh_ogs_gather_scatter_mixed(self.h, _lib.storage_of(q.dtype), k, q.ptr),
+               "spxh_ogs_gather_scatter_mixed")
 
     def gather(self, gq, q, k=1, op=_lib.ADD, trans=_lib.TRANS):
         checkh(libspxhost().spxh_ogs_gather(self.h, ogs_type(q.dtype), op, trans, k, q.ptr, gq.ptr),

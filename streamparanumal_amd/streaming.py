@@ -60,3 +60,25 @@ class Reducer:
         n = p.n if n is None else n
         check(libspx().spx_bs5_full(n, alpha, p.ptr, Ap.ptr, x.ptr, r.ptr, self.tmp.ptr, self.out.ptr,
                                     self.mode, self.block_size), "spx_bs5_full")
+
+
+class MixedCG:
+    """CG update on FP32 or BF16 (uint16 bits) vectors: x += alpha p and
+    r -= alpha Ap in FP32, rdotr = sum of the stored r^2 in FP64.  FP64
+    vectors use Reducer.cg_full."""
+
+    def __init__(self):
+        self.tmp = DeviceArray(_lib.MIXED_TMP_COUNT)
+        self.out = DeviceArray(1)
+
+    def value(self):
+        return float(self.out.to_numpy()[0])
+
+    def cg_full(self, alpha, p, Ap, x, r, n=None):
+        n = p.n if n is None else n
+        st = _lib.storage_of(p.dtype)
+        for a in (Ap, x, r):
+            if a.dtype != p.dtype:
+                raise TypeError("p, Ap, x, r must share one storage type")
+        check(libspx().spx_bs5_full_mixed(st, n, alpha, p.ptr, Ap.ptr, x.ptr, r.ptr, self.tmp.ptr,
+                                          self.out.ptr), "spx_bs5_full_mixed")

@@ -1,0 +1,157 @@
+"""GPU parity of the reduced-precision storage variants (FP32 / BF16) against the oracle.
+
+Gather+scatter: FP64 row sums over stored members in ascending local index,
+rounded once -- bit-exact against the oracle's restatement.  CG update: FP32
+operations without contraction, stored results bit-exact; rdotr (an FP64 sum
+in another order than the oracle's long double) within 1e-14 relative.
+The error of each variant against the FP64 path is checked to its bound.
+"""
+import numpy as np
+import pytest
+
+from streamparanumal_amd import STORE_BF16, STORE_F32, STORE_F64, device, ogs
+from streamparanumal_amd._lib import SpxError, check, libspx
+from streamparanumal_amd.device import DeviceArray
+from streamparanumal_amd.streaming import MixedCG, Reducer
+
+pytestmark = pytest.mark.gpu
+
+MESHES = [(1, 1, 1, 1), (2, 2, 2, 2), (3, 2, 1, 3), (5, 3, 2, 4), (4, 4, 4, 15), (8, 8, 8, 7)]
+STORES = [STORE_F32, STORE_BF16]
+NP = {STORE_F32: np.float32, STORE_BF16: np.uint16}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def setups(dev, oracle):
+    out = {}
+    for m in MESHES:
+        ids = ogs.hex_ids(*m)
+        out[m] = (ogs.Ogs(ids), oracle.Ogs(ids))
+    return out
+
+
+@pytest.mark.parametrize("mesh", MESHES)
+@pytest.mark.parametrize("st", STORES)
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_gs_mixed_bit_exact(setups, oracle, mesh, st, k):
+    say("gs mixed", mesh, st, k)
+    g, o = setups[mesh]
+    q = oracle.to_storage(oracle.uniform(g.N * k, 11), st)
+    dq = DeviceArray.from_numpy(q)
+    g.gather_scatter_mixed(dq, k)
+    device.check_kernel_errors()
+    want = oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, q, k)
+    assert np.array_equal(dq.to_numpy().view(np.uint32 if st == STORE_F32 else np.uint16),
+                          want.view(np.uint32 if st == STORE_F32 else np.uint16))
+
+
+@pytest.mark.parametrize("cap", [256, 1024, 2048, 4096])
+@pytest.mark.parametrize("st", STORES)
+def test_gs_mixed_block_caps(setups, oracle, cap, st):
+    say("gs mixed cap", cap, st)
+    g, o = setups[(8, 8, 8, 7)]
+    g.set_block_nnz(2048, gs_cap=cap)
+    try:
+        q = oracle.to_storage(oracle.uniform(g.N, 12), st)
+        dq = DeviceArray.from_numpy(q)
+        g.gather_scatter_mixed(dq)
+        device.check_kernel_errors()
+        assert np.array_equal(dq.to_numpy(), oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, q))
+    finally:
+        g.set_block_nnz(2048, gs_cap=4096)
+
+
+def test_gs_mixed_f64_is_plain(setups, oracle):
+    say("gs mixed f64")
+    g, o = setups[(5, 3, 2, 4)]
+    q = oracle.uniform(g.N, 13)
+    dq = DeviceArray.from_numpy(q)
+    g.gather_scatter_mixed(dq)
+    assert np.array_equal(dq.to_numpy().view(np.uint64), o.gather_scatter(q).view(np.uint64))
+
+
+@pytest.mark.parametrize("st,eps", [(STORE_F32, 2.0**-24), (STORE_BF16, 2.0**-8)])
+def test_gs_mixed_error_vs_f64(setups, oracle, st, eps):
+    say("gs mixed error", st)
+    g, o = setups[(8, 8, 8, 7)]
+    q = oracle.uniform(g.N, 14)
+    exact = o.gather_scatter(q)
+    dq = DeviceArray.from_numpy(oracle.to_storage(q, st))
+    g.gather_scatter_mixed(dq)
+    got = oracle.from_storage(dq.to_numpy(), st)
+    bound = eps * o.gather_scatter(np.abs(q)) + eps * np.abs(exact)
+    assert np.all(np.abs(got - exact) <= bound * (1 + 1e-12))
+    rel = np.linalg.norm(got - exact) / np.linalg.norm(exact)
+    print(f"[err] storage {st}: normwise relative error {rel:.3e}", flush=True)
+    if st == STORE_F32:
+        assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 1023, 4096, 100003, 1 << 20, 3_000_001])
+@pytest.mark.parametrize("st", STORES)
+def test_cg_mixed_bit_exact(dev, oracle, n, st):
+    say("cg mixed", n, st)
+    alpha = 0.4375 + 1e-9
+    vecs = [oracle.to_storage(oracle.uniform(n, s), st) for s in (21, 22, 23, 24)]
+    p, Ap, x, r = (DeviceArray.from_numpy(v) for v in vecs)
+    cg = MixedCG()
+    cg.cg_full(alpha, p, Ap, x, r)
+    v, xo, ro = oracle.cg_mixed(st, alpha, *vecs)
+    assert np.array_equal(x.to_numpy(), xo) and np.array_equal(r.to_numpy(), ro)
+    assert abs(cg.value() - v) <= 1e-14 * abs(v)
+
+
+def test_cg_mixed_empty_and_errors(dev, oracle):
+    say("cg mixed edge")
+    cg = MixedCG()
+    e = DeviceArray(4, np.float32)
+    cg.cg_full(1.0, e, e, e, e, n=0)
+    assert cg.value() == 0.0
+    a = DeviceArray(16, np.float32)
+    with pytest.raises(SpxError):  # misaligned: one entry in
+        check(libspx().spx_bs5_full_mixed(STORE_F32, 8, 1.0, a.offset(1), a.ptr, a.ptr, a.ptr, cg.tmp.ptr,
+                                          cg.out.ptr), "spx_bs5_full_mixed")
+    with pytest.raises(SpxError):
+        check(libspx().spx_bs5_full_mixed(STORE_F64, 8, 1.0, a.ptr, a.ptr, a.ptr, a.ptr, cg.tmp.ptr,
+                                          cg.out.ptr), "spx_bs5_full_mixed")
+
+
+@pytest.mark.parametrize("st,tol", [(STORE_F32, 1e-6), (STORE_BF16, 2e-2)])
+def test_cg_mixed_error_vs_f64(dev, oracle, st, tol):
+    say("cg mixed error", st)
+    n = 1 << 20
+    alpha = 0.31
+    vecs64 = [oracle.uniform(n, s) for s in (31, 32, 33, 34)]
+    p, Ap, x, r = (DeviceArray.from_numpy(v) for v in vecs64)
+    red = Reducer(n)
+    red.cg_full(alpha, p, Ap, x, r)
+    x64, r64, v64 = x.to_numpy(), r.to_numpy(), red.value()
+    ps, Aps, xs, rs = (DeviceArray.from_numpy(oracle.to_storage(v, st)) for v in vecs64)
+    cg = MixedCG()
+    cg.cg_full(alpha, ps, Aps, xs, rs)
+    xm, rm = oracle.from_storage(xs.to_numpy(), st), oracle.from_storage(rs.to_numpy(), st)
+    ex = np.linalg.norm(xm - x64) / np.linalg.norm(x64)
+    er = np.linalg.norm(rm - r64) / np.linalg.norm(r64)
+    ed = abs(cg.value() - v64) / v64
+    print(f"[err] storage {st}: x {ex:.3e} r {er:.3e} rdotr {ed:.3e}", flush=True)
+    assert max(ex, er, ed) <= tol
+
+
+@pytest.mark.parametrize("st", STORES)
+def test_device_conversions(dev, oracle, st):
+    say("convert", st)
+    n = 200003
+    u = oracle.uniform(n, 41) * np.exp2((oracle.rand_int(n, 42) % 40) - 20.0)
+    u[:4] = [0.0, -0.0, 1.0 + 2.0**-8 + 2.0**-40, 1.0 + 2.0**-8]
+    src = DeviceArray.from_numpy(u)
+    dst = DeviceArray(n, NP[st])
+    device.convert(src, dst)
+    want = oracle.to_storage(u, st)
+    assert np.array_equal(dst.to_numpy(), want)
+    back = DeviceArray(n)
+    device.convert(dst, back)
+    assert np.array_equal(back.to_numpy(), oracle.from_storage(want, st))

@@ -233,3 +233,43 @@ def test_full_size_bs8_64cube_p7(dev, oracle):
     device.check_kernel_errors()
     want = oracle.csr_gs(rs, ci, oracle.uniform(ids.size, 1))
     assert same(q.to_numpy(), want)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("trans", [spx.NOTRANS, spx.TRANS])
+@pytest.mark.parametrize("k", [1, 3, 6])
+def test_scatter_paths_agree(setups, dtype, trans, k):
+    """Scatter through the local->row map and over the CSR row blocks give the oracle's result."""
+    mesh = (5, 3, 2, 4)
+    say("scatter paths", mesh, np.dtype(dtype).name, trans, k)
+    g, o, _ = setups[mesh]
+    srows = g.NlocalP if trans == spx.TRANS else g.NlocalT
+    gsrc = values(srows, dtype, 8, k)
+    base = values(g.N, dtype, 9, k)
+    want = o.scatter(gsrc, base, k, trans)
+    try:
+        for by_rows in (False, True):
+            g.set_scatter_path(by_rows)
+            dv, dg = DeviceArray.from_numpy(base), DeviceArray.from_numpy(gsrc)
+            g.scatter(dv, dg, k, trans)
+            device.check_kernel_errors()
+            assert same(dv.to_numpy(), want), by_rows
+    finally:
+        g.set_scatter_path(True)
+
+
+def test_unflagged_scatter_leaves_zero_ids_alone(dev, oracle):
+    """Local nodes with id 0 belong to no row: both scatter paths leave them untouched."""
+    ids = ogs.hex_ids(3, 2, 2, 2)
+    ids[::7] = 0
+    g, o = ogs.Ogs(ids), oracle.Ogs(ids)
+    gsrc = oracle.uniform(g.NlocalT, 12)
+    base = oracle.uniform(g.N, 13)
+    want = o.scatter(gsrc, base, 1, spx.NOTRANS)
+    for by_rows in (False, True):
+        g.set_scatter_path(by_rows)
+        dv, dg = DeviceArray.from_numpy(base), DeviceArray.from_numpy(gsrc)
+        g.scatter(dv, dg, 1, spx.NOTRANS)
+        assert same(dv.to_numpy(), want)
+        assert same(dv.to_numpy()[::7], base[::7])
+    g.set_scatter_path(True)

@@ -312,3 +312,107 @@ def test_mul_max_min_ops():
     assert np.array_equal(og.gather_scatter(q, 1, 3, 0), [-1.0, -3.0, -1.0, 7.0, -3.0, -1.0])
     qi = np.array([3, 1, 4, 1, 5, 9], dtype=np.int32)
     assert np.array_equal(og.gather_scatter(qi, 1, 0, 0), [16, 6, 16, 1, 6, 16])
+
+
+# ---------------------------------------------------------------------------
+# reduced-precision storage (FP32 / BF16) restatements
+# ---------------------------------------------------------------------------
+
+def test_bf16_rounding_known_answers(oracle):
+    f = oracle.lib().oracle_bf16_from_double
+    assert f(1.0) == 0x3F80 and f(-2.0) == 0xC000 and f(0.0) == 0 and f(-0.0) == 0x8000
+    assert f(1.0 + 2.0**-8) == 0x3F80            # tie -> even (down)
+    assert f(1.0 + 3 * 2.0**-8) == 0x3F82        # tie -> even (up)
+    assert f(1.0 + 2.0**-8 + 2.0**-40) == 0x3F81  # above the tie: a f64->f32->bf16 chain gets 0x3F80
+    assert f(1.0 + 2.0**-8 - 2.0**-40) == 0x3F80
+    assert f(float("inf")) == 0x7F80 and f(-float("inf")) == 0xFF80
+    assert f(3.0e38) == 0x7F62                   # 3e38 -> 1.1015625 * 2^127 (nearest bf16)
+
+
+def test_bf16_from_float_matches_torch(oracle):
+    torch = pytest.importorskip("torch")
+    u = oracle.uniform(20000, 3) * np.exp2((oracle.rand_int(20000, 4) % 60) - 30.0)
+    f32 = u.astype(np.float32)
+    mine = oracle.to_storage(f32.astype(np.float64), oracle.STORE_BF16)   # f32 values are exact in f64
+    ref = torch.from_numpy(f32).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(mine, ref)
+    g = oracle.lib().oracle_bf16_from_float
+    assert all(g(float(x)) == int(y) for x, y in zip(f32[:500], ref[:500]))
+
+
+def test_bf16_from_double_brute_force_nearest(oracle):
+    # nearest-even by brute force: compare against the two bf16 neighbours in exact arithmetic
+    from fractions import Fraction
+    u = oracle.uniform(3000, 9) * np.exp2((oracle.rand_int(3000, 10) % 20) - 10.0)
+    got = oracle.to_storage(u, oracle.STORE_BF16)
+    back = oracle.from_storage(got, oracle.STORE_BF16)
+    for x, b, bits in zip(u, back, got):
+        e = int(np.frexp(abs(x))[1])             # |x| in [2^(e-1), 2^e)
+        ulp = Fraction(2) ** (e - 1 - 7)
+        fx, fb = Fraction(float(x)), Fraction(float(b))
+        assert abs(fx - fb) <= ulp / 2
+        if abs(fx - fb) == ulp / 2:
+            assert int(bits) % 2 == 0
+
+
+def test_gs_mixed_f64_storage_is_plain_gather_scatter(oracle):
+    ids = oracle.hex_ids(3, 2, 2, 3)
+    o = oracle.Ogs(ids)
+    q = oracle.uniform(ids.size, 1)
+    assert np.array_equal(oracle.gs_mixed(oracle.STORE_F64, o.rowStartsT, o.colIdsT, q), o.gather_scatter(q))
+
+
+def test_gs_mixed_rounds_once(oracle):
+    # members 1, 2^-24, 2^-24 in f32: a float fold drops both small terms (ties to even),
+    # one rounding of the f64 sum keeps them
+    rs = np.array([0, 3], dtype=np.int32)
+    ci = np.array([0, 1, 2], dtype=np.int32)
+    q = np.array([1.0, 2.0**-24, 2.0**-24], dtype=np.float32)
+    out = oracle.gs_mixed(oracle.STORE_F32, rs, ci, q)
+    assert np.all(out == np.float32(1.0 + 2.0**-23))
+    # bf16 (ulp 2^-7 at 1): 1 + 2 * 3*2^-10 rounds up; a bf16 fold loses each 3*2^-10 term
+    qb = oracle.to_storage(np.array([1.0, 3 * 2.0**-10, 3 * 2.0**-10]), oracle.STORE_BF16)
+    outb = oracle.from_storage(oracle.gs_mixed(oracle.STORE_BF16, rs, ci, qb), oracle.STORE_BF16)
+    assert np.all(outb == 1.0 + 2.0**-7)
+
+
+def test_gs_mixed_error_bound_vs_f64(oracle):
+    ids = oracle.hex_ids(4, 3, 3, 5)
+    o = oracle.Ogs(ids)
+    q = oracle.uniform(ids.size, 2)
+    exact = o.gather_scatter(q)
+    for st, eps in ((oracle.STORE_F32, 2.0**-24), (oracle.STORE_BF16, 2.0**-8)):
+        qs = oracle.to_storage(q, st)
+        got = oracle.from_storage(oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, qs), st)
+        # input rounding (<= eps |q_i| per member) plus one output rounding (<= eps |sum|)
+        absq = np.abs(q)
+        bound = eps * o.gather_scatter(absq) + eps * np.abs(exact) + 1e-300
+        assert np.all(np.abs(got - exact) <= bound * (1 + 1e-12))
+
+
+def test_cg_mixed_closed_form(oracle):
+    # dyadic values: every FP32 operation exact, so the result is the real-number update
+    n = 1001
+    p = ((oracle.rand_int(n, 1) % 64) - 32).astype(np.float32) / 8
+    Ap = ((oracle.rand_int(n, 2) % 64) - 32).astype(np.float32) / 16
+    x = ((oracle.rand_int(n, 3) % 64) - 32).astype(np.float32) / 4
+    r = ((oracle.rand_int(n, 4) % 64) - 32).astype(np.float32) / 2
+    v, xo, ro = oracle.cg_mixed(oracle.STORE_F32, 0.5, p, Ap, x, r)
+    xe = x.astype(np.float64) + 0.5 * p
+    re = r.astype(np.float64) - 0.5 * Ap
+    assert np.array_equal(xo, xe.astype(np.float32)) and np.array_equal(ro, re.astype(np.float32))
+    assert v == float(np.sum(re * re))
+
+
+def test_cg_mixed_no_fma_and_store_rounding(oracle):
+    # r - a*Ap with one rounding per operation: a*Ap = (1+2^-12)^2 rounds to 1+2^-11 in f32
+    a, ap = np.float32(1 + 2.0**-12), np.float32(1 + 2.0**-12)
+    p = np.zeros(1, np.float32)
+    v, _, ro = oracle.cg_mixed(oracle.STORE_F32, float(a), p, np.array([ap]), p.copy(), np.array([np.float32(1 + 2.0**-11)]))
+    assert ro[0] == 0.0 and v == 0.0   # a fused a*Ap would leave -2^-24
+    qb = oracle.to_storage(np.array([1.0]), oracle.STORE_BF16)
+    zb = oracle.to_storage(np.array([0.0]), oracle.STORE_BF16)
+    rb = oracle.to_storage(np.array([2.0**-9]), oracle.STORE_BF16)
+    v, xb, rb2 = oracle.cg_mixed(oracle.STORE_BF16, 1.0 + 2.0**-8, zb, zb, qb, rb)
+    assert oracle.from_storage(xb, oracle.STORE_BF16)[0] == 1.0   # x unchanged
+    assert oracle.from_storage(rb2, oracle.STORE_BF16)[0] == 2.0**-9 and v == 2.0**-18
