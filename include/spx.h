@@ -200,13 +200,48 @@ int spx_ogs_scatter_inverse(int type, int K, int32_t n, const int32_t *g2l, cons
 /* extract (ogsKernels.okl:167-177): gq[k + n*K] = q[k + ids[n]*K], n < n_ids */
 int spx_ogs_extract(int type, int K, int32_t n_ids, const int32_t *ids, const void *q, void *gq);
 
+/* Sorted-run row blocks: the T-CSR of a Sym operator re-listed for
+   coalesced access (no reference counterpart; the rows, and the member order
+   within each row, are those of the CSR of ogsOperator.hpp:44-70; built by
+   spxh_srb_build / the ogs_t setup).  Block b holds whole rows; its entries
+   [entryStarts[b], entryStarts[b+1]) are listed in ascending local index
+   ("positions" 0..n_b-1 of the block) and cut into chunks of 64: chunks
+   [chunkStarts[b], chunkStarts[b+1]), one per 64 positions.
+     runs:  bit l of runMask[c] is set when position 64c'+l (c' the chunk's
+            index within its block) starts a run of consecutive local indices
+            (bit 0 always); the chunk's runs begin at local indices
+            runIds[runBase[c]], runIds[runBase[c]+1], ...
+     rows:  bit l of linkMask[c] is set when that position is not the first
+            (smallest) member of its row; such positions, in order, have the
+            block position of the previous member of their row in
+            links[linkBase[c]], links[linkBase[c]+1], ...; links ends with
+            one padding word (read, never used, for chunks without links)
+   Row values are folded from the first member along these links, i.e. in
+   ascending local index, as the CSR kernel does.  Arrays are device memory. */
+typedef struct {
+  int32_t nblocks;
+  int32_t max_block_nnz; /* <= 4096 */
+  int32_t xcd_remap;     /* 1: consecutive blocks on one XCD, 0: hardware order, -1: library default */
+  const int32_t *entryStarts;
+  const int32_t *chunkStarts;
+  const uint64_t *runMask;
+  const int32_t *runBase;
+  const int32_t *runIds;
+  const uint64_t *linkMask;
+  const int32_t *linkBase;
+  const uint16_t *links;
+} spx_srb_t;
+/* gatherScatter, ogs::Sym (ogsKernels.okl:32-81), on the sorted-run layout:
+   same result as spx_ogs_gather_scatter on the CSR it was built from */
+int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, void *q);
+
 /* ------------------------------------------- reduced-precision storage -- */
 /* Storage types of the mixed-precision variants (no reference counterpart:
    the reference's dfloat is one type for storage and arithmetic,
    include/types.h:36-44 and README.md:49-53). */
 enum { SPX_STORE_F64 = 0, SPX_STORE_F32 = 1, SPX_STORE_BF16 = 2 };
 /* partials needed by spx_bs5_full_mixed's tmp */
-enum { SPX_MIXED_TMP_COUNT = 256 };
+enum { SPX_MIXED_TMP_COUNT = 2048 };
 
 /* ogs::Sym Add gatherScatter (ogsKernels.okl:32-81) on a vector stored as
    `storage`: each row's sum is taken in FP64 over its members in stored
@@ -215,6 +250,8 @@ enum { SPX_MIXED_TMP_COUNT = 256 };
 int spx_ogs_gather_scatter_mixed(int storage, int K, int32_t nblocks, const int32_t *blockStarts,
                                  const int32_t *rowStarts, const int32_t *colIds,
                                  int32_t max_block_nnz, void *q);
+/* the same on the sorted-run layout of spx_ogs_gather_scatter_srb */
+int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, void *q);
 /* CG update (README.md:49-53) on FP32 or BF16 vectors: x += alpha*p and
    r -= alpha*Ap in FP32 (alpha rounded to FP32), results rounded to the
    storage type; *rdotr = sum of the stored r squared, accumulated in FP64.

@@ -58,6 +58,10 @@ int spxh_ogs_set_gs_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
 /* scatter path: 1 = over the CSR row blocks (ogsKernels.okl scatter; the
    default), 0 = through the local->row map in local order */
 int spxh_ogs_set_scatter_path(spxh_ogs_t ogs, int by_rows);
+/* layout of the Sym gather+scatter: 1 = sorted-run row blocks (spx_srb_t, the
+   default), 0 = the T-CSR row blocks; results are identical.  xcd_remap is the
+   sorted-run launch hint (-1 library default, 0 hardware order, 1 remap). */
+int spxh_ogs_set_gs_format(spxh_ogs_t ogs, int sorted, int xcd_remap);
 /* ogs_t::SetupGlobalToLocalMapping (ogsSetup.cpp:833-849) */
 int spxh_ogs_global_to_local(spxh_ogs_t ogs, int32_t *g2l);
 
@@ -82,6 +86,21 @@ int spxh_csr_destroy(spxh_csr_t csr);
 /* row blocks of at most cap entries (greedy); blockStarts holds nrows+2 */
 int spxh_block_rows(int32_t nrows, const int32_t *rowStarts, int32_t cap, int32_t *nblocks,
                     int32_t *blockStarts);
+
+/* Sorted-run row blocks (spx_srb_t) of a CSR and its row blocks, on the host
+   (no device needed): sizes[0..5] = nblocks, nnz, nchunks, nruns, nlinks,
+   max_block_nnz.  spxh_srb_copy fills host arrays of nblocks+1 (entryStarts,
+   chunkStarts), nchunks (runMask, runBase, linkMask, linkBase), nruns
+   (runIds) and nlinks + 1 (links, the last a padding word) entries (any may
+   be NULL); the caller uploads them and fills spx_srb_t with the device
+   addresses. */
+typedef struct spxh_srb_s *spxh_srb_t;
+int spxh_srb_build(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int32_t nblocks,
+                   const int32_t *blockStarts, int32_t max_block_nnz, spxh_srb_t *out);
+int spxh_srb_sizes(spxh_srb_t srb, int64_t *sizes);
+int spxh_srb_copy(spxh_srb_t srb, int32_t *entryStarts, int32_t *chunkStarts, uint64_t *runMask,
+                  int32_t *runBase, int32_t *runIds, uint64_t *linkMask, int32_t *linkBase, uint16_t *links);
+int spxh_srb_destroy(spxh_srb_t srb);
 
 /* mesh_t hex box ids: lexicographic GLL-grid ids of an nx x ny x nz box at
    degree p, n = nx*ny*nz*(p+1)^3 entries (meshSetupBoxHex3D.cpp:87-130) */

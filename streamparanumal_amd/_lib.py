@@ -19,7 +19,7 @@ REDUCE_REFERENCE, REDUCE_FAST = 0, 1
 # storage types of the reduced-precision variants; BF16 vectors are held as
 # their raw 16 bits (numpy uint16)
 STORE_F64, STORE_F32, STORE_BF16 = 0, 1, 2
-MIXED_TMP_COUNT = 256
+MIXED_TMP_COUNT = 2048
 
 
 def storage_of(dtype):
@@ -55,6 +55,13 @@ class OgsInfo(C.Structure):
                 ("NblocksT", i32), ("NblocksN", i32), ("block_nnz", i32), ("gs_block_nnz", i32),
                 ("Ngs_blocks", i32), ("gather_defined", i32),
                 ("kind", i32), ("unique", i32), ("NgatherGlobal", i64), ("setup_seconds", dbl)]
+
+
+class Srb(C.Structure):
+    """spx_srb_t: device addresses of a sorted-run row block layout."""
+    _fields_ = [("nblocks", i32), ("max_block_nnz", i32), ("xcd_remap", i32), ("entryStarts", vp),
+                ("chunkStarts", vp), ("runMask", vp), ("runBase", vp), ("runIds", vp), ("linkMask", vp),
+                ("linkBase", vp), ("links", vp)]
 
 
 # name: (restype, argtypes) -- every function of include/spx.h
@@ -107,6 +114,8 @@ SPX_FUNCS = {
     "spx_ogs_gather_scatter": (C.c_int, [C.c_int, C.c_int, C.c_int, i32, vp, i32, vp, vp, i32, vp, vp, i32, vp]),
     "spx_ogs_scatter_inverse": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp]),
     "spx_ogs_extract": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp]),
+    "spx_ogs_gather_scatter_srb": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Srb), vp]),
+    "spx_ogs_gather_scatter_srb_mixed": (C.c_int, [C.c_int, C.c_int, C.POINTER(Srb), vp]),
     "spx_ogs_gather_scatter_mixed": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, i32, vp]),
     "spx_bs5_full_mixed": (C.c_int, [C.c_int, C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp]),
     "spx_set_tuning": (C.c_int, [cstr, C.POINTER(Tuning)]),
@@ -127,6 +136,11 @@ SPXH_FUNCS = {
     "spxh_ogs_global_to_local": (C.c_int, [vp, vp]),
     "spxh_ogs_gather_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "spxh_ogs_gather_scatter_mixed": (C.c_int, [vp, C.c_int, C.c_int, vp]),
+    "spxh_ogs_set_gs_format": (C.c_int, [vp, C.c_int, C.c_int]),
+    "spxh_srb_build": (C.c_int, [i32, vp, vp, i32, vp, i32, C.POINTER(vp)]),
+    "spxh_srb_sizes": (C.c_int, [vp, vp]),
+    "spxh_srb_copy": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "spxh_srb_destroy": (C.c_int, [vp]),
     "spxh_ogs_gather": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_ogs_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_csr_build": (C.c_int, [i32, vp, C.c_int, C.c_int, C.POINTER(vp)]),

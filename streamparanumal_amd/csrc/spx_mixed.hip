@@ -97,109 +97,174 @@ int launch_gs_mixed(const Launch &L, const int *bs, const int *st, const int *id
   return SPX_OK;
 }
 
+// ------------------------------------------ mixed gather+scatter, sorted runs --
+// FP64 row sums over the stored members, one rounding (k_gs_srb's fold policy).
+template <typename S> struct FoldWide {
+  typedef double Acc;
+  static __device__ __forceinline__ double init() { return 0.0; }
+  static __device__ __forceinline__ void apply(double &a, S x) { a += Store<S>::to_f64(x); }
+  static __device__ __forceinline__ S out(double a) { return Store<S>::from_f64(a); }
+};
+
+template <typename S>
+int launch_gs_srb_mixed(const Launch &L, const SrbArgs &A, S *q) {
+  Launch W = L;
+  const bool wave = L.variant == 2 && L.cap <= 16 * 64;
+  const int lanes = wave ? 64 : kThreads;
+  W.per = pick_per(W.kf, W.cap * (kThreads / lanes));
+  if (!W.per) {
+    W.kf = 1;
+    W.per = pick_per(1, W.cap * (kThreads / lanes));
+  }
+  if (!W.per) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile too large");
+  const size_t lds = (size_t)W.cap * W.kf * sizeof(S) + (size_t)W.cap * sizeof(unsigned short);
+  if (lds > 65536) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile too large for LDS");
+  const dim3 grid(W.nblocks, W.kf == 1 ? W.K : 1);
+#define SPX_GSRM(KF, PER)                                                                                   \
+  do {                                                                                                     \
+    if (wave)                                                                                              \
+      hipLaunchKernelGGL((k_gs_srb<S, SPX_ADD, KF, PER, 64, FoldWide<S>>), grid, dim3(64), lds, W.stream, W, \
+                         SRB_ARGS(A), q);                                                                  \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_gs_srb<S, SPX_ADD, KF, PER, kThreads, FoldWide<S>>), grid, dim3(kThreads), lds,  \
+                         W.stream, W, SRB_ARGS(A), q);                                                     \
+  } while (0)
+  if (W.kf == 1 && W.per == 4) SPX_GSRM(1, 4);
+  else if (W.kf == 1 && W.per == 8) SPX_GSRM(1, 8);
+  else if (W.kf == 1 && W.per == 16) SPX_GSRM(1, 16);
+  else if (W.kf == 2 && W.per == 4) SPX_GSRM(2, 4);
+  else if (W.kf == 2 && W.per == 8) SPX_GSRM(2, 8);
+  else if (W.kf == 3 && W.per == 4) SPX_GSRM(3, 4);
+  else if (W.kf == 4 && W.per == 4) SPX_GSRM(4, 4);
+  else return fail(SPX_ERR_UNSUPPORTED, "spx_ogs_gather_scatter_srb_mixed: no kernel for this tile");
+#undef SPX_GSRM
+  return SPX_OK;
+}
+
 // ------------------------------------------------------------ mixed CG update --
-// Chunks of 4 consecutive entries; lane t of block b takes chunks
-// t + b*256 + j*256*nblock in j order, U chunks in flight, FP64 sum.
-template <typename S> struct Vec4;
-template <> struct Vec4<float> {
-  typedef float4 type;
+// Chunks of 4 consecutive entries (one float4 or 8-byte bf16 quad per vector);
+// a fixed grid walks them with stride 256*gridDim, U chunks per lane in flight,
+// and folds each lane's r^2 in FP64.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+template <typename S> struct Quad;
+template <> struct Quad<float> {
+  typedef f32x4 type;
 };
-template <> struct Vec4<bf16_t> {
-  typedef ushort4 type;
+template <> struct Quad<bf16_t> {
+  typedef u32x2 type;
 };
 
-template <typename S>
-__device__ __forceinline__ void ld4(const S *p, int64_t c, float (&o)[4]) {
-  if constexpr (sizeof(S) == 4) {
-    const float4 v = reinterpret_cast<const float4 *>(p)[c];
-    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-  } else {
-    const ushort4 v = reinterpret_cast<const ushort4 *>(p)[c];
-    o[0] = bf16_bits_to_f32(v.x); o[1] = bf16_bits_to_f32(v.y);
-    o[2] = bf16_bits_to_f32(v.z); o[3] = bf16_bits_to_f32(v.w);
-  }
+template <bool NT, typename V> __device__ __forceinline__ V ldq(const V *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename V> __device__ __forceinline__ void stq(V *p, V v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
 }
 
-// stores the rounded values and returns them (as stored) in o
-template <typename S>
-__device__ __forceinline__ void st4(S *p, int64_t c, float (&o)[4]) {
-  if constexpr (sizeof(S) == 4) {
-    reinterpret_cast<float4 *>(p)[c] = make_float4(o[0], o[1], o[2], o[3]);
-  } else {
-    ushort4 v;
-    v.x = f32_to_bf16_bits(o[0]); v.y = f32_to_bf16_bits(o[1]);
-    v.z = f32_to_bf16_bits(o[2]); v.w = f32_to_bf16_bits(o[3]);
-    reinterpret_cast<ushort4 *>(p)[c] = v;
-    o[0] = bf16_bits_to_f32(v.x); o[1] = bf16_bits_to_f32(v.y);
-    o[2] = bf16_bits_to_f32(v.z); o[3] = bf16_bits_to_f32(v.w);
-  }
+__device__ __forceinline__ void unpack(const f32x4 &v, float (&o)[4]) {
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ void unpack(const u32x2 &v, float (&o)[4]) {
+  o[0] = __uint_as_float(v.x << 16);
+  o[1] = __uint_as_float(v.x & 0xffff0000u);
+  o[2] = __uint_as_float(v.y << 16);
+  o[3] = __uint_as_float(v.y & 0xffff0000u);
+}
+// round to the storage type; o receives the stored values
+__device__ __forceinline__ void pack(float (&o)[4], f32x4 &v) {
+  v.x = o[0]; v.y = o[1]; v.z = o[2]; v.w = o[3];
+}
+__device__ __forceinline__ void pack(float (&o)[4], u32x2 &v) {
+  const uint32_t b0 = f32_to_bf16_bits(o[0]), b1 = f32_to_bf16_bits(o[1]);
+  const uint32_t b2 = f32_to_bf16_bits(o[2]), b3 = f32_to_bf16_bits(o[3]);
+  v.x = b0 | (b1 << 16);
+  v.y = b2 | (b3 << 16);
+  o[0] = __uint_as_float(b0 << 16); o[1] = __uint_as_float(b1 << 16);
+  o[2] = __uint_as_float(b2 << 16); o[3] = __uint_as_float(b3 << 16);
 }
 
-constexpr int kCgBlocks = 256;
-constexpr int kCgU = 8;
+constexpr int kCgGridMax = 2048;  // partials written by stage 1
 
-template <typename S>
-__global__ __launch_bounds__(256) void k_cg_mixed1(int64_t n, float alpha, const S *__restrict__ p,
-                                                   const S *__restrict__ Ap, S *__restrict__ x, S *__restrict__ r,
-                                                   double *__restrict__ partials) {
+template <typename S, int U, bool NT>
+__global__ __launch_bounds__(256) void k_cg_mixed1(int64_t n, float alpha, const S *__restrict__ p_,
+                                                   const S *__restrict__ Ap_, S *__restrict__ x_,
+                                                   S *__restrict__ r_, double *__restrict__ partials) {
+  typedef typename Quad<S>::type V;
+  const V *__restrict__ p = reinterpret_cast<const V *>(p_);
+  const V *__restrict__ Ap = reinterpret_cast<const V *>(Ap_);
+  V *__restrict__ x = reinterpret_cast<V *>(x_);
+  V *__restrict__ r = reinterpret_cast<V *>(r_);
   __shared__ double s[256];
   const int t = threadIdx.x;
   const int64_t nchunk = n / 4;
   const int64_t stride = (int64_t)256 * gridDim.x;
   int64_t c = t + (int64_t)blockIdx.x * 256;
   double acc = 0.0;
-  for (; c + (int64_t)(kCgU - 1) * stride < nchunk; c += (int64_t)kCgU * stride) {
-    float vp[kCgU][4], vap[kCgU][4], vx[kCgU][4], vr[kCgU][4];
+  for (; c + (int64_t)(U - 1) * stride < nchunk; c += (int64_t)U * stride) {
+    V vp[U], vap[U], vx[U], vr[U];
 #pragma unroll
-    for (int u = 0; u < kCgU; ++u) {
-      ld4(p, c + u * stride, vp[u]);
-      ld4(Ap, c + u * stride, vap[u]);
-      ld4(x, c + u * stride, vx[u]);
-      ld4(r, c + u * stride, vr[u]);
+    for (int u = 0; u < U; ++u) {
+      vp[u] = ldq<NT>(p + c + u * stride);
+      vap[u] = ldq<NT>(Ap + c + u * stride);
+      vx[u] = ldq<NT>(x + c + u * stride);
+      vr[u] = ldq<NT>(r + c + u * stride);
     }
 #pragma unroll
-    for (int u = 0; u < kCgU; ++u) {
+    for (int u = 0; u < U; ++u) {
+      float fp[4], fap[4], fx[4], fr[4];
+      unpack(vp[u], fp);
+      unpack(vap[u], fap);
+      unpack(vx[u], fx);
+      unpack(vr[u], fr);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float ap = alpha * vp[u][e];
-        vx[u][e] = vx[u][e] + ap;
-        const float aAp = alpha * vap[u][e];
-        vr[u][e] = vr[u][e] - aAp;
+        const float ap = alpha * fp[e];
+        fx[e] = fx[e] + ap;
+        const float aAp = alpha * fap[e];
+        fr[e] = fr[e] - aAp;
       }
-      st4(x, c + u * stride, vx[u]);
-      st4(r, c + u * stride, vr[u]);
+      pack(fx, vx[u]);
+      pack(fr, vr[u]);
+      stq<NT>(x + c + u * stride, vx[u]);
+      stq<NT>(r + c + u * stride, vr[u]);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc += (double)vr[u][e] * (double)vr[u][e];
+      for (int e = 0; e < 4; ++e) acc += (double)fr[e] * (double)fr[e];
     }
   }
   for (; c < nchunk; c += stride) {
-    float vp[4], vap[4], vx[4], vr[4];
-    ld4(p, c, vp);
-    ld4(Ap, c, vap);
-    ld4(x, c, vx);
-    ld4(r, c, vr);
+    float fp[4], fap[4], fx[4], fr[4];
+    unpack(p[c], fp);
+    unpack(Ap[c], fap);
+    unpack(x[c], fx);
+    unpack(r[c], fr);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const float ap = alpha * vp[e];
-      vx[e] = vx[e] + ap;
-      const float aAp = alpha * vap[e];
-      vr[e] = vr[e] - aAp;
+      const float ap = alpha * fp[e];
+      fx[e] = fx[e] + ap;
+      const float aAp = alpha * fap[e];
+      fr[e] = fr[e] - aAp;
     }
-    st4(x, c, vx);
-    st4(r, c, vr);
+    V vx, vr;
+    pack(fx, vx);
+    pack(fr, vr);
+    x[c] = vx;
+    r[c] = vr;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) acc += (double)vr[e] * (double)vr[e];
+    for (int e = 0; e < 4; ++e) acc += (double)fr[e] * (double)fr[e];
   }
   // the n % 4 tail entries: lane 0 of block 0, in order
   if (blockIdx.x == 0 && t == 0) {
     for (int64_t i = nchunk * 4; i < n; ++i) {
-      const float ap = alpha * Store<S>::to_f32(p[i]);
-      const float xn = Store<S>::to_f32(x[i]) + ap;
-      const float aAp = alpha * Store<S>::to_f32(Ap[i]);
-      const float rn = Store<S>::to_f32(r[i]) - aAp;
-      x[i] = Store<S>::from_f32(xn);
-      r[i] = Store<S>::from_f32(rn);
-      const double rs = Store<S>::to_f64(r[i]);
+      const float ap = alpha * Store<S>::to_f32(p_[i]);
+      const float xn = Store<S>::to_f32(x_[i]) + ap;
+      const float aAp = alpha * Store<S>::to_f32(Ap_[i]);
+      const float rn = Store<S>::to_f32(r_[i]) - aAp;
+      x_[i] = Store<S>::from_f32(xn);
+      r_[i] = Store<S>::from_f32(rn);
+      const double rs = Store<S>::to_f64(r_[i]);
       acc += rs * rs;
     }
   }
@@ -213,11 +278,14 @@ __global__ __launch_bounds__(256) void k_cg_mixed1(int64_t n, float alpha, const
   if (t == 0) partials[blockIdx.x] = s[0];
 }
 
+// one block: lane t folds partials t, t+256, ... then a tree
 __global__ __launch_bounds__(256) void k_sum_partials(int nb, const double *__restrict__ partials,
                                                       double *__restrict__ out) {
   __shared__ double s[256];
   const int t = threadIdx.x;
-  s[t] = t < nb ? partials[t] : 0.0;
+  double a = 0.0;
+  for (int i = t; i < nb; i += 256) a += partials[i];
+  s[t] = a;
   __syncthreads();
 #pragma unroll
   for (int w = 128; w >= 1; w >>= 1) {
@@ -225,6 +293,38 @@ __global__ __launch_bounds__(256) void k_sum_partials(int nb, const double *__re
     __syncthreads();
   }
   if (t == 0) out[0] = s[0];
+}
+
+template <typename S>
+int launch_cg_mixed(int64_t n, float alpha, const S *p, const S *Ap, S *x, S *r, double *tmp, double *rdotr) {
+  // defaults measured on MI355X at n = 2^27: 512 blocks, streaming loads and
+  // stores, 4 chunks per lane for FP32 (5.56 TB/s), 2 for BF16 (6.02 TB/s)
+  spx_tuning_t d{};
+  d.unroll = sizeof(S) == 4 ? 4 : 2;
+  d.grid_blocks = 512;
+  d.nontemporal = 1;
+  d.xcd_remap = 0;
+  d.variant = 0;
+  const spx_tuning_t tu = tuning_for("cg_mixed", d);
+  const int U = tu.unroll;
+  const bool nt = tu.nontemporal > 0;
+  int64_t grid = tu.grid_blocks;
+  const int64_t need = (n / 4 + 255) / 256;
+  if (grid > need) grid = need;
+  if (grid < 1) grid = 1;
+  if (grid > kCgGridMax) grid = kCgGridMax;
+  const dim3 g((unsigned)grid);
+#define SPX_CG(UU, NTT) \
+  hipLaunchKernelGGL((k_cg_mixed1<S, UU, NTT>), g, dim3(256), 0, current_stream(), n, alpha, p, Ap, x, r, tmp)
+  if (U == 1) { if (nt) SPX_CG(1, true); else SPX_CG(1, false); }
+  else if (U == 2) { if (nt) SPX_CG(2, true); else SPX_CG(2, false); }
+  else if (U == 4) { if (nt) SPX_CG(4, true); else SPX_CG(4, false); }
+  else if (U == 8) { if (nt) SPX_CG(8, true); else SPX_CG(8, false); }
+  else return fail(SPX_ERR_ARG, "spx_bs5_full_mixed: unroll must be 1, 2, 4 or 8");
+#undef SPX_CG
+  SPX_LAUNCHED("spx_bs5_full_mixed");
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, current_stream(), (int)grid, tmp, rdotr);
+  return SPX_OK;
 }
 
 }  // namespace
@@ -264,6 +364,45 @@ int spx_ogs_gather_scatter_mixed(int storage, int K, int32_t nblocks, const int3
   return SPX_OK;
 }
 
+int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, void *q) {
+  SPX_NEED_DEVICE();
+  if (!srb) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: null layout");
+  if (storage == SPX_STORE_F64) return spx_ogs_gather_scatter_srb(SPX_DOUBLE, SPX_ADD, K, srb, q);
+  if (storage != SPX_STORE_F32 && storage != SPX_STORE_BF16)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: unknown storage type");
+  if (K < 1 || K > 65535) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: bad K");
+  if (srb->nblocks < 0 || srb->max_block_nnz < 1 || srb->max_block_nnz > 4096)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: bad blocks");
+  if (srb->nblocks == 0) return SPX_OK;
+  if (!srb->entryStarts || !srb->chunkStarts || !srb->runMask || !srb->runBase || !srb->runIds ||
+      !srb->linkMask || !srb->linkBase || !srb->links || !q)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: null pointer");
+  const spx_tuning_t tu = tuning_for("ogs_gs_srb", spx_tuning_t{0, 1, 0, 0, 0, 2});
+  Launch L;
+  L.nblocks = srb->nblocks;
+  L.K = K;
+  L.nxcd = (srb->xcd_remap >= 0 ? srb->xcd_remap : tu.xcd_remap) ? 8 : 1;
+  L.cap = srb->max_block_nnz;
+  L.kf = (K >= 2 && K <= 4) ? K : 1;
+  L.per = 0;
+  L.variant = tu.variant;
+  L.stream = current_stream();
+  L.err = kernel_error_flag();
+  SrbArgs A;
+  A.entryStarts = srb->entryStarts;
+  A.chunkStarts = srb->chunkStarts;
+  A.runMask = reinterpret_cast<const unsigned long long *>(srb->runMask);
+  A.runBase = srb->runBase;
+  A.runIds = srb->runIds;
+  A.linkMask = reinterpret_cast<const unsigned long long *>(srb->linkMask);
+  A.linkBase = srb->linkBase;
+  A.links = srb->links;
+  const int rc = storage == SPX_STORE_F32 ? launch_gs_srb_mixed(L, A, (float *)q) : launch_gs_srb_mixed(L, A, (bf16_t *)q);
+  if (rc) return rc;
+  SPX_LAUNCHED("spx_ogs_gather_scatter_srb_mixed");
+  return SPX_OK;
+}
+
 int spx_bs5_full_mixed(int storage, int64_t n, double alpha, const void *p, const void *Ap, void *x, void *r,
                        double *tmp, double *rdotr) {
   SPX_NEED_DEVICE();
@@ -275,16 +414,12 @@ int spx_bs5_full_mixed(int storage, int64_t n, double alpha, const void *p, cons
   const size_t al = storage == SPX_STORE_F32 ? 16 : 8;
   if (!aligned(p, al) || !aligned(Ap, al) || !aligned(x, al) || !aligned(r, al))
     return fail(SPX_ERR_ARG, "spx_bs5_full_mixed: vectors must be aligned to 4 entries");
-  int64_t nb = (n / 4 + 255) / 256;
-  nb = nb < 1 ? 1 : (nb > kCgBlocks ? kCgBlocks : nb);
-  if (storage == SPX_STORE_F32)
-    hipLaunchKernelGGL(k_cg_mixed1<float>, dim3((unsigned)nb), dim3(256), 0, current_stream(), n, (float)alpha,
-                       (const float *)p, (const float *)Ap, (float *)x, (float *)r, tmp);
-  else
-    hipLaunchKernelGGL(k_cg_mixed1<bf16_t>, dim3((unsigned)nb), dim3(256), 0, current_stream(), n, (float)alpha,
-                       (const bf16_t *)p, (const bf16_t *)Ap, (bf16_t *)x, (bf16_t *)r, tmp);
-  SPX_LAUNCHED("spx_bs5_full_mixed");
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, current_stream(), (int)nb, tmp, rdotr);
+  int rc = storage == SPX_STORE_F32
+               ? launch_cg_mixed(n, (float)alpha, (const float *)p, (const float *)Ap, (float *)x, (float *)r, tmp,
+                                 rdotr)
+               : launch_cg_mixed(n, (float)alpha, (const bf16_t *)p, (const bf16_t *)Ap, (bf16_t *)x, (bf16_t *)r,
+                                 tmp, rdotr);
+  if (rc) return rc;
   SPX_LAUNCHED("spx_bs5_full_mixed");
   return SPX_OK;
 }

@@ -21,14 +21,17 @@ size_t type_size(int type) {
   }
 }
 
-// Measured on MI355X (BS8, 64^3 hex at p=7): 16-bit row starts in LDS
-// (variant 3) 4.9-5.0 TB/s against 4.4-5.0 for ranges in registers, both
-// across 2048- and 4096-entry tiles; XCD remap within run-to-run noise.
+// Measured on MI355X (BS8, 64^3 hex at p=7).  CSR blocks: 16-bit row starts
+// in LDS (variant 3) 4.9-5.0 TB/s against 4.4-5.0 for ranges in registers.
+// Sorted-run blocks: one-wave tiles (variant 2) with XCD remap, 5.1 TB/s at
+// p=7 and 5.8 at p=15, ahead of 256-lane tiles (variant 0) at every degree
+// measured; the remap gains 5-40% there.
 spx_tuning_t ogs_defaults(const char *kname) {
   spx_tuning_t d{};
+  const std::string k(kname);
   d.block_threads = kThreads;
-  d.xcd_remap = 0;
-  d.variant = std::string(kname) == "ogs_gs" ? 3 : 0;
+  d.xcd_remap = k == "ogs_gs_srb" ? 1 : 0;
+  d.variant = k == "ogs_gs" ? 3 : k == "ogs_gs_srb" ? 2 : 0;
   return d;
 }
 
@@ -145,6 +148,37 @@ int spx_ogs_gather_scatter(int type, int op, int K, int32_t nblocks, const int32
   });
   if (rc) return rc;
   SPX_LAUNCHED("spx_ogs_gather_scatter");
+  return SPX_OK;
+}
+
+int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, void *q) {
+  SPX_NEED_DEVICE();
+  if (!srb) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: null layout");
+  int rc = check_common("spx_ogs_gather_scatter_srb", type, K, srb->nblocks, srb->max_block_nnz);
+  if (rc || (rc = check_op("spx_ogs_gather_scatter_srb", op))) return rc;
+  if (srb->nblocks == 0) return SPX_OK;
+  if (!srb->entryStarts || !srb->chunkStarts || !srb->runMask || !srb->runBase || !srb->runIds ||
+      !srb->linkMask || !srb->linkBase || !srb->links || !q)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: null pointer");
+  Launch L;
+  if ((rc = make_launch("spx_ogs_gather_scatter_srb", "ogs_gs_srb", srb->nblocks, K, srb->max_block_nnz, type, 1, L)))
+    return rc;
+  if (srb->xcd_remap >= 0) L.nxcd = srb->xcd_remap ? 8 : 1;
+  SrbArgs A;
+  A.entryStarts = srb->entryStarts;
+  A.chunkStarts = srb->chunkStarts;
+  A.runMask = reinterpret_cast<const unsigned long long *>(srb->runMask);
+  A.runBase = srb->runBase;
+  A.runIds = srb->runIds;
+  A.linkMask = reinterpret_cast<const unsigned long long *>(srb->linkMask);
+  A.linkBase = srb->linkBase;
+  A.links = srb->links;
+  rc = by_type(type, [&](auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return launch_gs_srb(L, op, A, (T *)q);
+  });
+  if (rc) return rc;
+  SPX_LAUNCHED("spx_ogs_gather_scatter_srb");
   return SPX_OK;
 }
 

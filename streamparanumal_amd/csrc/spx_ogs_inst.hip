@@ -55,6 +55,32 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
   });
 }
 
+int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q) {
+  const size_t lds = lds_bytes(L.cap, L.kf, sizeof(T), 1) + (size_t)L.cap * sizeof(unsigned short);
+  if (lds > 65536) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: tile too large for LDS");
+  if (L.variant == 2 && L.cap <= 16 * 64) {
+    // one-wave tiles: 64 lanes, cap <= 64*PER; fields one per pass when a
+    // multi-field tile does not fit one wave's registers (larger tiles take
+    // the 256-lane kernel below)
+    Launch W = L;
+    W.per = pick_per(W.kf, W.cap * (kThreads / 64));
+    if (!W.per) {
+      W.kf = 1;
+      W.per = pick_per(1, W.cap * (kThreads / 64));
+    }
+    if (!W.per) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: tile too large for one wave");
+    const size_t ldsw = lds_bytes(W.cap, W.kf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short);
+    return dispatch<true>(op, W.kf, W.per, [&]<int OPC, int KF, int PER>() {
+      hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), ldsw, W.stream, W, SRB_ARGS(A), q);
+      return SPX_OK;
+    });
+  }
+  return dispatch<true>(op, L.kf, L.per, [&]<int OPC, int KF, int PER>() {
+    hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER>), grid_of(L), dim3(kThreads), lds, L.stream, L, SRB_ARGS(A), q);
+    return SPX_OK;
+  });
+}
+
 int launch_gs_gen(const Launch &L, int op, int ngrows, int nsrows, const int *bs, const int *gs, const int *gi,
                   const int *ss, const int *si, T *q) {
   const size_t lds = lds_bytes(L.cap, L.kf, sizeof(T), 2);

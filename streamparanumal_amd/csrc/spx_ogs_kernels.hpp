@@ -55,23 +55,32 @@ template <> struct Lim<double> { static __device__ __forceinline__ double max() 
 template <> struct Lim<int> { static __device__ __forceinline__ int max() { return INT_MAX; } };
 template <> struct Lim<long long> { static __device__ __forceinline__ long long max() { return LLONG_MAX; } };
 
-// ogsOperator.cpp:37-56 (host) / ogsUtils.cpp:62-103 (device defines)
+// ogsOperator.cpp:37-56 (host) / ogsUtils.cpp:62-103 (device defines).
+// Acc is the accumulator type, out() turns it into the stored result.
 template <typename T, int OP> struct Fold;
 template <typename T> struct Fold<T, SPX_ADD> {
+  typedef T Acc;
   static __device__ __forceinline__ T init() { return T(0); }
   static __device__ __forceinline__ void apply(T &a, T b) { a += b; }
+  static __device__ __forceinline__ T out(T a) { return a; }
 };
 template <typename T> struct Fold<T, SPX_MUL> {
+  typedef T Acc;
   static __device__ __forceinline__ T init() { return T(1); }
   static __device__ __forceinline__ void apply(T &a, T b) { a *= b; }
+  static __device__ __forceinline__ T out(T a) { return a; }
 };
 template <typename T> struct Fold<T, SPX_MAX> {
+  typedef T Acc;
   static __device__ __forceinline__ T init() { return -Lim<T>::max(); }
   static __device__ __forceinline__ void apply(T &a, T b) { a = (b > a) ? b : a; }
+  static __device__ __forceinline__ T out(T a) { return a; }
 };
 template <typename T> struct Fold<T, SPX_MIN> {
+  typedef T Acc;
   static __device__ __forceinline__ T init() { return Lim<T>::max(); }
   static __device__ __forceinline__ void apply(T &a, T b) { a = (b < a) ? b : a; }
+  static __device__ __forceinline__ T out(T a) { return a; }
 };
 
 __device__ __forceinline__ void flag_error(int *err, int code) {
@@ -79,6 +88,7 @@ __device__ __forceinline__ void flag_error(int *err, int code) {
 }
 
 __device__ __forceinline__ int clampi(int x, int hi) { return x < hi ? x : hi; }
+
 
 // Column ids of entries t, t+256, ... (clamped to the block's last entry).
 template <int PER>
@@ -202,6 +212,118 @@ __global__ __launch_bounds__(kThreads) void k_gs_sym(Launch L, const int *__rest
         if constexpr (VAR == 2) __builtin_nontemporal_store(x, dst);
         else *dst = x;
       }
+}
+
+// ----------------------------------------------------- Sym GS, sorted runs --
+// The block's entries in ascending local index ("positions"): lane loads and
+// stores hit runs of consecutive nodes.  Position pos, chunk c = pos/64,
+// l = pos%64, decodes as
+//   local index = runIds[runBase[c] + popc(runMask[c] & bits<=l) - 1] + (l - top)
+//     where top is the highest set bit of runMask[c] at or below l;
+//   previous member of its row (when bit l of linkMask[c] is set)
+//              = links[linkBase[c] + popc(linkMask[c] & bits<l)].
+// Values sit in LDS at their positions; next[] links each member to the
+// following one, and the first member of each row folds the row along them
+// (ascending local index, the CSR order) and writes the result back.  NT = 64
+// gives one-wave tiles: more independent tiles per CU, no cross-wave barrier.
+struct SrbArgs {
+  const int *entryStarts, *chunkStarts;
+  const unsigned long long *runMask, *linkMask;
+  const int *runBase, *runIds, *linkBase;
+  const unsigned short *links;
+};
+
+constexpr unsigned short kEnd = 0xffff;
+
+// The layout arrays as separate restrict kernel arguments: the compiler may
+// then read the block's uniform words through the scalar cache even though
+// the kernel stores to q.
+#define SRB_PARAMS                                                                                            \
+  const int *__restrict__ entryStarts, const int *__restrict__ chunkStarts,                                    \
+      const unsigned long long *__restrict__ runMask, const int *__restrict__ runBase,                         \
+      const int *__restrict__ runIds, const unsigned long long *__restrict__ linkMask,                         \
+      const int *__restrict__ linkBase, const unsigned short *__restrict__ links, T *__restrict__ q
+#define SRB_ARGS(A) A.entryStarts, A.chunkStarts, A.runMask, A.runBase, A.runIds, A.linkMask, A.linkBase, A.links
+
+template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>>
+__global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS) {
+  const SrbArgs A{entryStarts, chunkStarts, runMask, linkMask, runBase, runIds, linkBase, links};
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *vals = reinterpret_cast<T *>(lds_raw);
+  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int kb = blockIdx.y * KF;
+  const int e0 = A.entryStarts[b];
+  const int nnz = A.entryStarts[b + 1] - e0;
+  const int c0 = A.chunkStarts[b];
+  const int nch = A.chunkStarts[b + 1] - c0;
+  if (nnz > L.cap || nch != (nnz + 63) / 64) {
+    flag_error(L.err, 6);
+    return;
+  }
+  if (nnz == 0) return;
+  const int64_t K = L.K;
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz) next[t + j * NT] = kEnd;
+  unsigned long long rm[PER], lm[PER];
+  int rb[PER], lb[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int c = c0 + min((NT / 64) * j + w, nch - 1);  // wave-uniform
+    rm[j] = A.runMask[c];
+    lm[j] = A.linkMask[c];
+    rb[j] = A.runBase[c];
+    lb[j] = A.linkBase[c];
+  }
+  int id[PER], prev[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int l = clampi(t + j * NT, nnz - 1) & 63;
+    const unsigned long long upto = (2ull << l) - 1ull;
+    const unsigned long long rbel = rm[j] & upto;
+    const int k = __builtin_popcountll(rbel) - 1;
+    const int top = 63 - __builtin_clzll(rbel);
+    id[j] = A.runIds[rb[j] + k] + (l - top);
+    const bool linked = (lm[j] >> l) & 1ull;
+    const int li = lb[j] + __builtin_popcountll(lm[j] & (upto >> 1));
+    prev[j] = linked ? (int)A.links[li] : -1;
+  }
+  T v[PER][KF];
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+#pragma unroll
+    for (int f = 0; f < KF; ++f) v[j][f] = q[(int64_t)id[j] * K + kb + f];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz) {
+#pragma unroll
+      for (int f = 0; f < KF; ++f) vals[(t + j * NT) * KF + f] = v[j][f];
+      if (prev[j] >= 0 && prev[j] < nnz) next[prev[j]] = (unsigned short)(t + j * NT);
+    }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz && prev[j] < 0) {
+      const int h = t + j * NT;
+#pragma unroll
+      for (int f = 0; f < KF; ++f) {
+        typename F::Acc acc = F::init();
+        // walks end at kEnd (> cap); the step bound also ends them on a malformed layout
+        for (int i = h, s = 0; i < nnz && s < nnz; i = next[i], ++s) F::apply(acc, vals[i * KF + f]);
+        const T r = F::out(acc);
+        for (int i = h, s = 0; i < nnz && s < nnz; i = next[i], ++s) vals[i * KF + f] = r;
+      }
+    }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) q[(int64_t)id[j] * K + kb + f] = vals[(t + j * NT) * KF + f];
 }
 
 // ------------------------------------------------------------ general GS --
@@ -457,7 +579,8 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
   int launch_gather(const Launch &L, int op, const int *bs, const int *st, const int *ids, const T *v, T *gv); \
   int launch_scatter(const Launch &L, const int *bs, const int *st, const int *ids, const T *gv, T *v);    \
   int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq);           \
-  int launch_scatter_inv(hipStream_t s, int64_t n, int K, const int *g2l, const T *gv, T *v, int npl);
+  int launch_scatter_inv(hipStream_t s, int64_t n, int K, const int *g2l, const T *gv, T *v, int npl); \
+  int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q);
 SPX_OGS_GEN_DECL(float)
 SPX_OGS_GEN_DECL(double)
 SPX_OGS_GEN_DECL(int)

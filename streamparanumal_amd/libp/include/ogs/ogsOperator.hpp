@@ -11,12 +11,32 @@ namespace ogs {
 
 // Reference defaults (ogsOperator.hpp:37-38) and this build's default tiles,
 // measured on MI355X at 64^3 hex p=7: gather/scatter fastest with 2048-entry
-// row blocks, gather+scatter with 4096.
+// row blocks, the sorted-run gather+scatter with 512-entry (one-wave) tiles.
 constexpr int blockSize = 256;
 constexpr int gatherNodesPerBlock = 512;
 constexpr int defaultBlockNnz = 2048;
-constexpr int defaultGsBlockNnz = 4096;
+constexpr int defaultGsBlockNnz = 512;
 constexpr int maxBlockNnz = 4096;
+
+// Sorted-run row blocks of the T-CSR (spx_srb_t): each row block's entries
+// in ascending local index, coded per 64-entry chunk as runs of consecutive
+// indices, with a link from each row member to the previous one.
+struct SortedRunBlocks {
+  dlong nblocks = 0, cap = 0, nnz = 0, nchunks = 0, nruns = 0, nlinks = 0;
+  int xcdRemap = -1;  // launch hint passed in spx_srb_t (-1: library default)
+  memory<dlong> entryStarts, chunkStarts, runBase, runIds, linkBase;
+  memory<uint64_t> runMask, linkMask;
+  memory<uint16_t> links;
+  deviceMemory<dlong> o_entryStarts, o_chunkStarts, o_runBase, o_runIds, o_linkBase;
+  deviceMemory<uint64_t> o_runMask, o_linkMask;
+  deviceMemory<uint16_t> o_links;
+
+  void Upload(platform_t &platform);
+  spx_srb_t Device() const;
+};
+
+SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowStarts, const memory<dlong> colIds,
+                                     const dlong Nblocks, const memory<dlong> blockStarts, const int cap);
 
 class ogsOperator_t {
  public:
@@ -40,6 +60,11 @@ class ogsOperator_t {
   dlong NgsBlocks = 0;
   memory<dlong> gsBlockRowStarts;
   deviceMemory<dlong> o_gsBlockRowStarts;
+
+  // The Sym gather+scatter runs on the sorted-run layout of the gs row
+  // blocks (gsSorted) or directly on the CSR.
+  SortedRunBlocks gsSrb;
+  bool gsSorted = true;
 
   // Local -> gathered row maps of the T- and N-CSR (-1: in no row), for the
   // scatter in local order (spx_ogs_scatter_inverse).  Measured on MI355X at

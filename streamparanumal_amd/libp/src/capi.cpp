@@ -157,6 +157,17 @@ int spxh_ogs_set_scatter_path(spxh_ogs_t h, int by_rows) {
   });
 }
 
+int spxh_ogs_set_gs_format(spxh_ogs_t h, int sorted, int xcd_remap) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_format: null");
+  if (xcd_remap < -1 || xcd_remap > 1) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_format: xcd_remap in {-1, 0, 1}");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    op->gsSorted = sorted != 0;
+    op->gsSrb.xcdRemap = xcd_remap;
+  });
+}
+
 int spxh_ogs_global_to_local(spxh_ogs_t h, int32_t *g2l) {
   if (!h || !g2l) return err(SPX_ERR_ARG, "spxh_ogs_global_to_local: null");
   return guarded([&] {
@@ -262,6 +273,67 @@ int spxh_csr_copy(spxh_csr_t h, int which, int32_t *rowStarts, int32_t *colIds) 
 }
 
 int spxh_csr_destroy(spxh_csr_t h) {
+  delete h;
+  return SPX_OK;
+}
+
+struct spxh_srb_s {
+  ogs::SortedRunBlocks srb;
+};
+
+int spxh_srb_build(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int32_t nblocks,
+                   const int32_t *blockStarts, int32_t cap, spxh_srb_t *out) {
+  if (!out || !rowStarts || !blockStarts || nrows < 0 || nblocks < 0) return err(SPX_ERR_ARG, "spxh_srb_build: bad arguments");
+  *out = nullptr;
+  return guarded([&] {
+    LIBP_ABORT("blockStarts must run from row 0 to row nrows",
+               blockStarts[0] != 0 || blockStarts[nblocks] != nrows);
+    for (int32_t b = 0; b < nblocks; ++b)
+      LIBP_ABORT("blockStarts must be nondecreasing", blockStarts[b + 1] < blockStarts[b]);
+    auto h = std::make_unique<spxh_srb_s>();
+    memory<dlong> rs(nrows + 1), bs(nblocks + 1);
+    std::copy(rowStarts, rowStarts + nrows + 1, rs.ptr());
+    std::copy(blockStarts, blockStarts + nblocks + 1, bs.ptr());
+    const dlong nnz = rs[nrows];
+    LIBP_ABORT("colIds missing", nnz > 0 && !colIds);
+    memory<dlong> ci(nnz > 0 ? nnz : 1);
+    if (nnz > 0) std::copy(colIds, colIds + nnz, ci.ptr());
+    h->srb = ogs::BuildSortedRunBlocks(nrows, rs, ci, nblocks, bs, cap);
+    *out = h.release();
+  });
+}
+
+int spxh_srb_sizes(spxh_srb_t h, int64_t *sizes) {
+  if (!h || !sizes) return err(SPX_ERR_ARG, "spxh_srb_sizes: null");
+  const ogs::SortedRunBlocks &s = h->srb;
+  sizes[0] = s.nblocks;
+  sizes[1] = s.nnz;
+  sizes[2] = s.nchunks;
+  sizes[3] = s.nruns;
+  sizes[4] = s.nlinks;
+  sizes[5] = s.cap;
+  return SPX_OK;
+}
+
+int spxh_srb_copy(spxh_srb_t h, int32_t *entryStarts, int32_t *chunkStarts, uint64_t *runMask, int32_t *runBase,
+                  int32_t *runIds, uint64_t *linkMask, int32_t *linkBase, uint16_t *links) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_srb_copy: null");
+  const ogs::SortedRunBlocks &s = h->srb;
+  auto cp = [](const auto &m, size_t n, auto *dst) {
+    if (dst && n) std::copy(m.ptr(), m.ptr() + n, dst);
+  };
+  cp(s.entryStarts, (size_t)s.nblocks + 1, entryStarts);
+  cp(s.chunkStarts, (size_t)s.nblocks + 1, chunkStarts);
+  cp(s.runMask, (size_t)s.nchunks, runMask);
+  cp(s.runBase, (size_t)s.nchunks, runBase);
+  cp(s.runIds, (size_t)s.nruns, runIds);
+  cp(s.linkMask, (size_t)s.nchunks, linkMask);
+  cp(s.linkBase, (size_t)s.nchunks, linkBase);
+  cp(s.links, (size_t)s.nlinks + 1, links);
+  return SPX_OK;
+}
+
+int spxh_srb_destroy(spxh_srb_t h) {
   delete h;
   return SPX_OK;
 }

@@ -71,6 +71,10 @@ void ogsOperator_t::SetGsBlockNnz(int nnz) {
   gsBlockNnz = nnz;
   blockRowsGreedy(NrowsT, rowStartsT, gsBlockNnz, NgsBlocks, gsBlockRowStarts);
   o_gsBlockRowStarts = platform.malloc<dlong>(NgsBlocks + 1, gsBlockRowStarts);
+  const int xcd = gsSrb.xcdRemap;
+  gsSrb = BuildSortedRunBlocks(NrowsT, rowStartsT, colIdsT, NgsBlocks, gsBlockRowStarts, gsBlockNnz);
+  gsSrb.xcdRemap = xcd;
+  gsSrb.Upload(platform);
 }
 
 void ogsOperator_t::SetBlockNnz(int nnz) {
@@ -134,7 +138,10 @@ void ogsOperator_t::ScatterRaw(Type type, Transpose trans, int k, const void *gv
 // (:411-427); here the device honours op as the API documents.
 void ogsOperator_t::GatherScatterRaw(Type type, Op op, Transpose trans, int k, void *v) {
   if (trans == Sym) {
-    if (NgsBlocks)
+    if (NgsBlocks && gsSorted) {
+      const spx_srb_t d = gsSrb.Device();
+      SPX_CHECK(spx_ogs_gather_scatter_srb((int)type, (int)op, k, &d, v));
+    } else if (NgsBlocks)
       SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NgsBlocks, o_gsBlockRowStarts.ptr(), NrowsT,
                                        o_rowStartsT.ptr(), o_colIdsT.ptr(), NrowsT, o_rowStartsT.ptr(),
                                        o_colIdsT.ptr(), gsBlockNnz, v));
@@ -152,7 +159,10 @@ void ogsOperator_t::GatherScatterRaw(Type type, Op op, Transpose trans, int k, v
 }
 
 void ogsOperator_t::GatherScatterMixedRaw(int storage, int k, void *v) {
-  if (NgsBlocks)
+  if (NgsBlocks && gsSorted) {
+    const spx_srb_t d = gsSrb.Device();
+    SPX_CHECK(spx_ogs_gather_scatter_srb_mixed(storage, k, &d, v));
+  } else if (NgsBlocks)
     SPX_CHECK(spx_ogs_gather_scatter_mixed(storage, k, NgsBlocks, o_gsBlockRowStarts.ptr(), o_rowStartsT.ptr(),
                                            o_colIdsT.ptr(), gsBlockNnz, v));
 }

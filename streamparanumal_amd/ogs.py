@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import checkh, libspxhost
+from ._lib import check, checkh, libspx, libspxhost
 from .device import ogs_type
 
 _platform_open = False
@@ -55,6 +55,46 @@ def build_csr(ids, kind=_lib.SIGNED, unique=True):
         return ids, out
     finally:
         libspxhost().spxh_csr_destroy(h)
+
+
+SRB_KEYS = ("entryStarts", "chunkStarts", "runMask", "runBase", "runIds", "linkMask", "linkBase", "links")
+
+
+def srb_build(row_starts, col_ids, block_starts, cap):
+    """Sorted-run row blocks (spx_srb_t) of a CSR and its row blocks, as host arrays."""
+    rs = np.ascontiguousarray(row_starts, dtype=np.int32)
+    ci = np.ascontiguousarray(col_ids, dtype=np.int32)
+    bs = np.ascontiguousarray(block_starts, dtype=np.int32)
+    h = C.c_void_p()
+    checkh(libspxhost().spxh_srb_build(rs.size - 1, rs.ctypes.data, ci.ctypes.data if ci.size else None,
+                                       bs.size - 1, bs.ctypes.data, cap, C.byref(h)), "spxh_srb_build")
+    try:
+        sz = np.zeros(6, dtype=np.int64)
+        checkh(libspxhost().spxh_srb_sizes(h, sz.ctypes.data), "spxh_srb_sizes")
+        nb, nnz, nch, nruns, nlinks, capv = (int(x) for x in sz)
+        out = {"entryStarts": np.zeros(nb + 1, np.int32), "chunkStarts": np.zeros(nb + 1, np.int32),
+               "runMask": np.zeros(max(nch, 1), np.uint64), "runBase": np.zeros(max(nch, 1), np.int32),
+               "runIds": np.zeros(max(nruns, 1), np.int32), "linkMask": np.zeros(max(nch, 1), np.uint64),
+               "linkBase": np.zeros(max(nch, 1), np.int32), "links": np.zeros(nlinks + 1, np.uint16)}
+        checkh(libspxhost().spxh_srb_copy(h, *(out[k].ctypes.data for k in SRB_KEYS)), "spxh_srb_copy")
+        out.update(nblocks=nb, nnz=nnz, nchunks=nch, nruns=nruns, nlinks=nlinks, max_block_nnz=capv)
+        return out
+    finally:
+        libspxhost().spxh_srb_destroy(h)
+
+
+class SrbDevice:
+    """A sorted-run layout uploaded to the device; `.desc` is the spx_srb_t to pass by reference."""
+
+    def __init__(self, srb, xcd_remap=-1):
+        from .device import DeviceArray
+        self.arrays = {k: DeviceArray.from_numpy(srb[k]) for k in SRB_KEYS}
+        self.desc = _lib.Srb(srb["nblocks"], srb["max_block_nnz"], xcd_remap,
+                             *(self.arrays[k].ptr for k in SRB_KEYS))
+
+    def gather_scatter(self, q, k=1, op=_lib.ADD):
+        check(libspx().spx_ogs_gather_scatter_srb(ogs_type(q.dtype), op, k, C.byref(self.desc), q.ptr),
+              "spx_ogs_gather_scatter_srb")
 
 
 def block_rows(row_starts, cap):
@@ -105,6 +145,12 @@ class Ogs:
         checkh(libspxhost().spxh_ogs_set_block_nnz(self.h, cap), "spxh_ogs_set_block_nnz")
         checkh(libspxhost().spxh_ogs_set_gs_block_nnz(self.h, gs_cap or cap), "spxh_ogs_set_gs_block_nnz")
         self.refresh()
+
+    def set_gs_format(self, sorted_runs, xcd_remap=-1):
+        """Sym gather+scatter on the sorted-run row blocks (True, default) or the T-CSR blocks (False);
+        xcd_remap: sorted-run launch hint (-1 library default)."""
+        checkh(libspxhost().spxh_ogs_set_gs_format(self.h, 1 if sorted_runs else 0, xcd_remap),
+               "spxh_ogs_set_gs_format")
 
     def set_scatter_path(self, by_rows):
         """Scatter over the CSR row blocks (True, default) or through the local->row map (False)."""

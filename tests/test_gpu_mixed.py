@@ -37,12 +37,17 @@ def setups(dev, oracle):
 @pytest.mark.parametrize("mesh", MESHES)
 @pytest.mark.parametrize("st", STORES)
 @pytest.mark.parametrize("k", [1, 2, 3, 4])
-def test_gs_mixed_bit_exact(setups, oracle, mesh, st, k):
-    say("gs mixed", mesh, st, k)
+@pytest.mark.parametrize("sorted_runs", [True, False])
+def test_gs_mixed_bit_exact(setups, oracle, mesh, st, k, sorted_runs):
+    say("gs mixed", mesh, st, k, "sorted-run" if sorted_runs else "csr")
     g, o = setups[mesh]
     q = oracle.to_storage(oracle.uniform(g.N * k, 11), st)
     dq = DeviceArray.from_numpy(q)
-    g.gather_scatter_mixed(dq, k)
+    g.set_gs_format(sorted_runs)
+    try:
+        g.gather_scatter_mixed(dq, k)
+    finally:
+        g.set_gs_format(True)
     device.check_kernel_errors()
     want = oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, q, k)
     assert np.array_equal(dq.to_numpy().view(np.uint32 if st == STORE_F32 else np.uint16),
@@ -62,7 +67,7 @@ def test_gs_mixed_block_caps(setups, oracle, cap, st):
         device.check_kernel_errors()
         assert np.array_equal(dq.to_numpy(), oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, q))
     finally:
-        g.set_block_nnz(2048, gs_cap=4096)
+        g.set_block_nnz(2048, gs_cap=512)
 
 
 def test_gs_mixed_f64_is_plain(setups, oracle):
@@ -74,11 +79,17 @@ def test_gs_mixed_f64_is_plain(setups, oracle):
     assert np.array_equal(dq.to_numpy().view(np.uint64), o.gather_scatter(q).view(np.uint64))
 
 
+def inputs(oracle, kind, n, seed):
+    """The two input sets of the precision study: U[-1,1), and |x| log-uniform in [1e-6, 1] with random sign."""
+    return oracle.uniform(n, seed) if kind == "uniform" else oracle.loguniform(n, seed)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "loguniform"])
 @pytest.mark.parametrize("st,eps", [(STORE_F32, 2.0**-24), (STORE_BF16, 2.0**-8)])
-def test_gs_mixed_error_vs_f64(setups, oracle, st, eps):
-    say("gs mixed error", st)
+def test_gs_mixed_error_vs_f64(setups, oracle, st, eps, kind):
+    say("gs mixed error", st, kind)
     g, o = setups[(8, 8, 8, 7)]
-    q = oracle.uniform(g.N, 14)
+    q = inputs(oracle, kind, g.N, 3)
     exact = o.gather_scatter(q)
     dq = DeviceArray.from_numpy(oracle.to_storage(q, st))
     g.gather_scatter_mixed(dq)
@@ -86,7 +97,7 @@ def test_gs_mixed_error_vs_f64(setups, oracle, st, eps):
     bound = eps * o.gather_scatter(np.abs(q)) + eps * np.abs(exact)
     assert np.all(np.abs(got - exact) <= bound * (1 + 1e-12))
     rel = np.linalg.norm(got - exact) / np.linalg.norm(exact)
-    print(f"[err] storage {st}: normwise relative error {rel:.3e}", flush=True)
+    print(f"[err] gs storage {st} {kind}: normwise relative error {rel:.3e}", flush=True)
     if st == STORE_F32:
         assert rel <= 1e-6
 
@@ -120,12 +131,13 @@ def test_cg_mixed_empty_and_errors(dev, oracle):
                                           cg.out.ptr), "spx_bs5_full_mixed")
 
 
+@pytest.mark.parametrize("kind", ["uniform", "loguniform"])
 @pytest.mark.parametrize("st,tol", [(STORE_F32, 1e-6), (STORE_BF16, 2e-2)])
-def test_cg_mixed_error_vs_f64(dev, oracle, st, tol):
-    say("cg mixed error", st)
+def test_cg_mixed_error_vs_f64(dev, oracle, st, tol, kind):
+    say("cg mixed error", st, kind)
     n = 1 << 20
     alpha = 0.31
-    vecs64 = [oracle.uniform(n, s) for s in (31, 32, 33, 34)]
+    vecs64 = [inputs(oracle, kind, n, s) for s in (3, 32, 33, 34)]
     p, Ap, x, r = (DeviceArray.from_numpy(v) for v in vecs64)
     red = Reducer(n)
     red.cg_full(alpha, p, Ap, x, r)
@@ -137,7 +149,7 @@ def test_cg_mixed_error_vs_f64(dev, oracle, st, tol):
     ex = np.linalg.norm(xm - x64) / np.linalg.norm(x64)
     er = np.linalg.norm(rm - r64) / np.linalg.norm(r64)
     ed = abs(cg.value() - v64) / v64
-    print(f"[err] storage {st}: x {ex:.3e} r {er:.3e} rdotr {ed:.3e}", flush=True)
+    print(f"[err] cg storage {st} {kind}: x {ex:.3e} r {er:.3e} rdotr {ed:.3e}", flush=True)
     assert max(ex, er, ed) <= tol
 
 

@@ -118,7 +118,7 @@ def test_all_types_ops_trans_fields(setups, dtype, op, trans, k):
 
 
 @pytest.mark.parametrize("cap", [64, 512, 1024, 3000, 4096])
-@pytest.mark.parametrize("variant", [0, 1, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, "srb"])
 def test_result_independent_of_row_blocks(setups, oracle, cap, variant):
     mesh = (5, 3, 2, 4)
     say("block cap", mesh, cap, "variant", variant)
@@ -127,7 +127,8 @@ def test_result_independent_of_row_blocks(setups, oracle, cap, variant):
     want = o.gather_scatter(q)
     gq_want = o.gather(q, 1, spx.ADD, spx.TRANS)
     g.set_block_nnz(cap)
-    device.set_tuning("ogs_gs", variant=variant)
+    g.set_gs_format(variant == "srb")
+    device.set_tuning("ogs_gs", variant=-1 if variant == "srb" else variant)
     try:
         dq = DeviceArray.from_numpy(q)
         g.gather_scatter(dq)
@@ -143,8 +144,59 @@ def test_result_independent_of_row_blocks(setups, oracle, cap, variant):
         device.check_kernel_errors()
         assert same(dout.to_numpy(), o.scatter(gq_want, np.zeros(g.N), 1, spx.NOTRANS))
     finally:
-        g.set_block_nnz(2048, 4096)
+        g.set_block_nnz(2048, 512)
+        g.set_gs_format(True)
         device.set_tuning("ogs_gs", variant=-1)
+
+
+@pytest.mark.parametrize("mesh", MESHES)
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("variant,cap,xcd", [(0, 1024, 0), (0, 4096, 1), (2, 1024, 1), (2, 4096, 0),
+                                             (2, 512, 0), (2, 256, 1)])
+def test_srb_equals_csr_format(setups, oracle, mesh, dtype, variant, cap, xcd):
+    """The sorted-run kernels (256-lane and one-wave tiles) and the T-CSR blocks give the same bits,
+    for every op and K = 1, 3."""
+    say("srb vs csr", mesh, dtype.__name__, variant, cap, xcd)
+    g, o, _ = setups[mesh]
+    g.set_block_nnz(2048, gs_cap=cap)
+    device.set_tuning("ogs_gs_srb", variant=variant)
+    try:
+        for k in (1, 3):
+            q = values(g.N, dtype, 23, k)
+            for op in OPS:
+                res = []
+                for fmt in (True, False):
+                    g.set_gs_format(fmt, xcd)
+                    dq = DeviceArray.from_numpy(q)
+                    g.gather_scatter(dq, k, op)
+                    device.check_kernel_errors()
+                    res.append(dq.to_numpy())
+                assert same(res[0], res[1])
+                assert same(res[0], o.gather_scatter(q, k, op))
+    finally:
+        g.set_gs_format(True, -1)
+        g.set_block_nnz(2048, gs_cap=512)
+        device.set_tuning("ogs_gs_srb", variant=-1)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("cap", [64, 700, 4096])
+def test_srb_direct_random_ids(dev, oracle, seed, cap):
+    """The C-ABI entry on a layout built from a CSR of random ids (runs of every length)."""
+    say("srb direct", seed, cap)
+    n = 20000
+    ids = (oracle.rand_int(n, 100 + seed) % 6000 + 1).astype(np.int64)
+    lone = oracle.rand_int(n, 200 + seed) % 5 == 0
+    ids[lone] = 10**6 + np.arange(np.count_nonzero(lone))           # a share of single-member rows
+    _, c = ogs.build_csr(ids)
+    rs, ci = c["rowStartsT"], c["colIdsT"]
+    bs = ogs.block_rows(rs, cap)
+    d = ogs.SrbDevice(ogs.srb_build(rs, ci, bs, cap))
+    q = oracle.uniform(n, seed)
+    dq = DeviceArray.from_numpy(q)
+    d.gather_scatter(dq)
+    device.check_kernel_errors()
+    assert same(dq.to_numpy(), oracle.csr_gs(rs, ci, q))
 
 
 def test_reference_bisection_blocks(setups, oracle):

@@ -4,6 +4,7 @@ import pytest
 
 import streamparanumal_amd as spx
 from streamparanumal_amd import ogs
+from streamparanumal_amd._lib import SpxError
 from oracle import pyoracle as o
 
 
@@ -78,3 +79,76 @@ def test_row_larger_than_cap_is_refused():
     _, csr = ogs.build_csr(ogs.hex_ids(2, 2, 2, 1))
     with pytest.raises(spx.SpxError):
         ogs.block_rows(csr["rowStartsT"], 4)
+
+
+# ---------------------------------------------------------------------------
+# sorted-run row blocks: decoded on the CPU and compared with the CSR
+# ---------------------------------------------------------------------------
+
+def decode_srb(s):
+    """Local index and previous-member position of every block position, by the device kernel's rule."""
+    ids = np.zeros(s["nnz"], np.int64)
+    prev = np.full(s["nnz"], -1, np.int64)
+    for b in range(s["nblocks"]):
+        e0, e1 = s["entryStarts"][b], s["entryStarts"][b + 1]
+        c0 = s["chunkStarts"][b]
+        for pos in range(e1 - e0):
+            c, l = c0 + (pos >> 6), pos & 63
+            rm, lm = int(s["runMask"][c]), int(s["linkMask"][c])
+            below = rm & ((2 << l) - 1)
+            k = bin(below).count("1") - 1
+            top = below.bit_length() - 1
+            ids[e0 + pos] = s["runIds"][s["runBase"][c] + k] + (l - top)
+            if lm >> l & 1:
+                prev[e0 + pos] = s["links"][s["linkBase"][c] + bin(lm & ((1 << l) - 1)).count("1")]
+    return ids, prev
+
+
+@pytest.mark.parametrize("mesh,cap", [((2, 2, 2, 2), 64), ((3, 2, 1, 3), 100), ((4, 3, 2, 4), 4096),
+                                      ((5, 5, 5, 1), 256), ((2, 2, 2, 7), 1000)])
+def test_srb_layout_matches_csr(mesh, cap):
+    ids = ogs.hex_ids(*mesh)
+    _, c = ogs.build_csr(ids)
+    rs, ci = c["rowStartsT"], c["colIdsT"]
+    bs = ogs.block_rows(rs, cap)
+    s = ogs.srb_build(rs, ci, bs, cap)
+    assert s["nblocks"] == bs.size - 1 and s["nnz"] == ci.size
+    assert np.array_equal(s["entryStarts"], rs[bs])
+    assert s["nlinks"] == ci.size - (rs.size - 1)            # every member but the first of each row
+    dec, prev = decode_srb(s)
+    for b in range(bs.size - 1):
+        e0, e1 = rs[bs[b]], rs[bs[b + 1]]
+        n = e1 - e0
+        assert s["chunkStarts"][b + 1] - s["chunkStarts"][b] == (n + 63) // 64
+        assert np.array_equal(dec[e0:e1], np.sort(ci[e0:e1]))          # ascending local index
+        # following the links from each row's first member walks the CSR row in order
+        pos_of = {int(x): i for i, x in enumerate(dec[e0:e1])}
+        nxt = {}
+        for i in range(n):
+            if prev[e0 + i] >= 0:
+                assert prev[e0 + i] < i and int(prev[e0 + i]) not in nxt
+                nxt[int(prev[e0 + i])] = i
+        for r in range(bs[b], bs[b + 1]):
+            row = ci[rs[r]:rs[r + 1]]
+            i = pos_of[int(row[0])]
+            assert prev[e0 + i] == -1
+            walk = [int(dec[e0 + i])]
+            while i in nxt:
+                i = nxt[i]
+                walk.append(int(dec[e0 + i]))
+            assert walk == [int(x) for x in row]
+    for c in range(s["nchunks"]):
+        assert int(s["runMask"][c]) & 1
+
+
+def test_srb_rejects_bad_blocks():
+    rs = np.array([0, 2, 3], np.int32)
+    ci = np.array([0, 1, 2], np.int32)
+    with pytest.raises(SpxError):
+        ogs.srb_build(rs, ci, np.array([0, 2], np.int32), 2)            # a block of 3 entries, cap 2
+    with pytest.raises(SpxError):
+        ogs.srb_build(rs, ci, np.array([0, 1], np.int32), 4)            # blocks stop short of the rows
+    with pytest.raises(SpxError):
+        ogs.srb_build(rs, np.array([0, 0, 2], np.int32), np.array([0, 2], np.int32), 4)  # node twice
+    empty = ogs.srb_build(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(1, np.int32), 16)
+    assert empty["nblocks"] == 0 and empty["nnz"] == 0
