@@ -76,6 +76,85 @@ def cpu_baseline():
                       f"{r['seconds'] * 1e3:.2f} ms/call, {ng / r['seconds'] / 1e9:.4f} GDOF/s"}
 
 
+def batched_suite(K, W, count=64, seed=4, fields=3):
+    """64 independent hex meshes (per-side elements U{4..48}, p in {3,5,7,9}, from the LCG stream of
+    `seed`), `fields` fields each: FP32-stored gather+scatter (FP64 row sums) then the CG update
+    with an FP64 rdotr per mesh, as one captured graph with autotuned launch shapes; against the
+    same work as 64 separate plain FP64 launch sequences, and as the batched graph in FP64."""
+    import numpy as np
+
+    import streamparanumal_amd as spx
+    from streamparanumal_amd import device, streaming
+    from streamparanumal_amd._lib import check, libspx
+    from streamparanumal_amd.batch import Batch, SeparateFP64, instances
+    from streamparanumal_amd.device import DeviceArray
+
+    alpha = 0.37
+    t0 = time.perf_counter()
+    meshes = instances(count, seed)
+    b = Batch(meshes, fields, np.float32)
+    setup_s = time.perf_counter() - t0
+    dofs = fields * b.global_dofs
+    log(f"batched: {count} meshes, {b.N} local / {b.global_dofs} global nodes, setup {setup_s:.1f}s")
+    b.allocate(seed=5)
+    t0 = time.perf_counter()
+    tune = b.autotune(reps=3)
+    log("batched autotune", f"{time.perf_counter() - t0:.1f}s", json.dumps(tune))
+
+    # error of one step against the plain FP64 path (bit-exact with the oracle), from the same inputs
+    b.allocate(seed=5)
+    sep = SeparateFP64(b, seed=5)
+    b.step(alpha)
+    sep.step(alpha)
+    device.finish()
+    nmax = max(fields * g.N for g in b.parts)
+    tmp = DeviceArray(nmax)
+    red = streaming.Reducer(nmax, spx.REDUCE_FAST, 256)
+
+    def sq(ptr, n):
+        check(libspx().spx_bs3_norm(n, ptr, red.tmp.ptr, red.out.ptr, spx.REDUCE_FAST, 256), "spx_bs3_norm")
+        return red.value()
+
+    worst = {"q": 0.0, "x": 0.0, "r": 0.0, "rdotr": 0.0}
+    rd32 = b.plan.values()
+    rd64 = sep.rdotr()
+    for i, g in enumerate(b.parts):
+        n, off = fields * g.N, int(b.seg_starts[i])
+        for key in ("q", "x", "r"):
+            ref = sep.vecs[i][key]
+            check(libspx().spx_convert_f32_to_f64(n, b.vectors[key].offset(off), tmp.ptr), "convert")
+            nref = sq(ref.ptr, n)
+            check(libspx().spx_bs2_axpy(n, 1.0, ref.ptr, -1.0, tmp.ptr), "axpy")
+            worst[key] = max(worst[key], math.sqrt(sq(tmp.ptr, n) / nref))
+        worst["rdotr"] = max(worst["rdotr"], abs(rd32[i] - rd64[i]) / rd64[i])
+    del tmp, red
+    log("batched error vs FP64 (max over meshes)", json.dumps(worst))
+
+    graph = b.capture(alpha)
+    h32, e32 = timed(lambda: graph.launch(), K, W)
+    h64s, e64s = timed(lambda: sep.step(alpha), K, W)
+    del sep, graph
+    b64 = b.sibling(np.float64)
+    b64.allocate(seed=5)
+    tune64 = b64.autotune(reps=3)
+    graph64 = b64.capture(alpha)
+    h64b, e64b = timed(lambda: graph64.launch(), K, W)
+    del graph64, b64
+    out = {"meshes": count, "fields": fields, "local_nodes": b.N, "global_nodes": b.global_dofs,
+           "setup_s": round(setup_s, 1),
+           "autotuned": tune, "autotuned_f64": tune64,
+           "batched_f32_graph": {"ms": round(h32 * 1e3, 3), "kernel_ms": round(e32 * 1e3, 3),
+                                 "GDOFs": round(dofs / h32 / 1e9, 2)},
+           "batched_f64_graph": {"ms": round(h64b * 1e3, 3), "GDOFs": round(dofs / h64b / 1e9, 2)},
+           "separate_f64": {"launches": 3 * count, "ms": round(h64s * 1e3, 3),
+                            "GDOFs": round(dofs / h64s / 1e9, 2)},
+           "speedup_f32_batched_vs_separate_f64": round(h64s / h32, 3),
+           "rel_err_vs_f64_max": {k: float(f"{v:.3e}") for k, v in worst.items()}}
+    log("batched", json.dumps(out))
+    del b
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +164,7 @@ def main():
     ap.add_argument("--degree", type=int, default=7)
     ap.add_argument("--log2n", type=int, default=27, help="BS1-BS5 vector length 2^log2n")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-mesh batched comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -249,6 +329,10 @@ def main():
     suite["BS0_launch"] = {"launches": nl, "plain_us_per_launch": round(plain * 1e6, 3),
                            "graph_us_per_launch": round(replay * 1e6, 3)}
     log("BS0", json.dumps(suite["BS0_launch"]))
+
+    # ------------------------------------------------------- batched instances
+    if not args.no_batched:
+        suite["batched_64_meshes"] = batched_suite(K, W)
 
     # ---------------------------------------------------------------- report
     value = bytes_bs8 / host8 / 1e9

@@ -222,6 +222,7 @@ typedef struct {
   int32_t nblocks;
   int32_t max_block_nnz; /* <= 4096 */
   int32_t xcd_remap;     /* 1: consecutive blocks on one XCD, 0: hardware order, -1: library default */
+  int32_t tile_lanes;    /* lanes per block: 64 (one-wave tiles, max_block_nnz <= 1024) or 256; 0: library default */
   const int32_t *entryStarts;
   const int32_t *chunkStarts;
   const uint64_t *runMask;
@@ -252,6 +253,21 @@ int spx_ogs_gather_scatter_mixed(int storage, int K, int32_t nblocks, const int3
                                  int32_t max_block_nnz, void *q);
 /* the same on the sorted-run layout of spx_ogs_gather_scatter_srb */
 int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, void *q);
+/* Batched CG update (README.md:49-53) over nseg independent vectors laid out
+   back to back ("segments": segment s is entries [segStarts[s],
+   segStarts[s+1]), each start a multiple of 4), in one launch plus one
+   per-segment fold: x += alpha*p, r -= alpha*Ap (FP64 storage: in FP64;
+   FP32 / BF16: in FP32, alpha rounded to FP32), rdotr[s] = sum over segment s
+   of the stored r squared, in FP64.  No reference counterpart (the reference
+   runs one vector per launch, BS/BS5).  A plan fixes the split of segments
+   into workgroups (about target_blocks in all, in proportion to length); the
+   partials are folded in a fixed order, so results do not vary run to run. */
+typedef struct spx_cg_plan_s *spx_cg_plan_t;
+int spx_cg_plan_create(int32_t nseg, const int64_t *segStarts, int32_t target_blocks, spx_cg_plan_t *plan);
+int spx_cg_plan_destroy(spx_cg_plan_t plan);
+int spx_cg_plan_blocks(spx_cg_plan_t plan);
+int spx_cg_batch(int storage, spx_cg_plan_t plan, double alpha, const void *p, const void *Ap, void *x,
+                 void *r, double *rdotr);
 /* CG update (README.md:49-53) on FP32 or BF16 vectors: x += alpha*p and
    r -= alpha*Ap in FP32 (alpha rounded to FP32), results rounded to the
    storage type; *rdotr = sum of the stored r squared, accumulated in FP64.

@@ -48,6 +48,11 @@ int spxh_platform_open(int device_id);
 int spxh_ogs_setup(int32_t n, int64_t *ids, int kind, int unique, int verbose, spxh_ogs_t *out);
 int spxh_ogs_destroy(spxh_ogs_t ogs);
 int spxh_ogs_info(spxh_ogs_t ogs, spxh_ogs_info_t *info);
+/* The block-diagonal union of nparts set-up handles as one new handle (no
+   reference counterpart): part i's local nodes follow parts 0..i-1's, so a
+   batch of independent meshes is gathered in one launch.  The parts stay
+   valid and independent. */
+int spxh_ogs_setup_batch(const spxh_ogs_t *parts, int32_t nparts, spxh_ogs_t *out);
 /* host copies of the operator: which = 0 (T-CSR) or 1 (N-CSR); arrays sized
    rows+1, nnz and blocks+1 (any may be NULL) */
 int spxh_ogs_copy_csr(spxh_ogs_t ogs, int which, int32_t *rowStarts, int32_t *colIds, int32_t *blockStarts);
@@ -62,6 +67,29 @@ int spxh_ogs_set_scatter_path(spxh_ogs_t ogs, int by_rows);
    default), 0 = the T-CSR row blocks; results are identical.  xcd_remap is the
    sorted-run launch hint (-1 library default, 0 hardware order, 1 remap). */
 int spxh_ogs_set_gs_format(spxh_ogs_t ogs, int sorted, int xcd_remap);
+/* Autotuner (no reference counterpart; the reference's ogs::Auto times its
+   halo exchange methods, ogsSetup.cpp:116-125, the same way): times every
+   candidate launch shape of the Sym gather+scatter -- layout, block
+   capacity, tile lanes, XCD remap -- on this operator for k fields of `type`
+   (storage = -1) or of mixed storage `storage` (SPX_STORE_*), keeps the
+   fastest and reports it.  Results never change with the shape. */
+typedef struct {
+  int32_t sorted, block_nnz, tile_lanes, xcd_remap;
+  double ms;
+  int32_t candidates;
+} spxh_gs_tuning_t;
+int spxh_ogs_autotune(spxh_ogs_t ogs, int type, int storage, int k, int reps, spxh_gs_tuning_t *best);
+/* Times spx_cg_batch over plans of several sizes and unroll / streaming
+   settings on the caller's vectors (alpha = 0: values unchanged); leaves
+   the best in the "cg_batch" tuning and returns its plan (the caller
+   destroys it). */
+typedef struct {
+  int32_t target_blocks, unroll, nontemporal;
+  double ms;
+  int32_t candidates;
+} spxh_cg_tuning_t;
+int spxh_cg_batch_autotune(int32_t nseg, const int64_t *segStarts, int storage, void *p, void *Ap, void *x,
+                           void *r, double *rdotr, int reps, spx_cg_plan_t *plan, spxh_cg_tuning_t *best);
 /* ogs_t::SetupGlobalToLocalMapping (ogsSetup.cpp:833-849) */
 int spxh_ogs_global_to_local(spxh_ogs_t ogs, int32_t *g2l);
 

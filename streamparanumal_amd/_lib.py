@@ -57,9 +57,18 @@ class OgsInfo(C.Structure):
                 ("kind", i32), ("unique", i32), ("NgatherGlobal", i64), ("setup_seconds", dbl)]
 
 
+class GsTuning(C.Structure):
+    _fields_ = [("sorted", i32), ("block_nnz", i32), ("tile_lanes", i32), ("xcd_remap", i32), ("ms", dbl),
+                ("candidates", i32)]
+
+
+class CgTuning(C.Structure):
+    _fields_ = [("target_blocks", i32), ("unroll", i32), ("nontemporal", i32), ("ms", dbl), ("candidates", i32)]
+
+
 class Srb(C.Structure):
     """spx_srb_t: device addresses of a sorted-run row block layout."""
-    _fields_ = [("nblocks", i32), ("max_block_nnz", i32), ("xcd_remap", i32), ("entryStarts", vp),
+    _fields_ = [("nblocks", i32), ("max_block_nnz", i32), ("xcd_remap", i32), ("tile_lanes", i32), ("entryStarts", vp),
                 ("chunkStarts", vp), ("runMask", vp), ("runBase", vp), ("runIds", vp), ("linkMask", vp),
                 ("linkBase", vp), ("links", vp)]
 
@@ -117,6 +126,10 @@ SPX_FUNCS = {
     "spx_ogs_gather_scatter_srb": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Srb), vp]),
     "spx_ogs_gather_scatter_srb_mixed": (C.c_int, [C.c_int, C.c_int, C.POINTER(Srb), vp]),
     "spx_ogs_gather_scatter_mixed": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, i32, vp]),
+    "spx_cg_plan_create": (C.c_int, [i32, vp, i32, C.POINTER(vp)]),
+    "spx_cg_plan_destroy": (C.c_int, [vp]),
+    "spx_cg_plan_blocks": (C.c_int, [vp]),
+    "spx_cg_batch": (C.c_int, [C.c_int, vp, C.c_double, vp, vp, vp, vp, vp]),
     "spx_bs5_full_mixed": (C.c_int, [C.c_int, C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp]),
     "spx_set_tuning": (C.c_int, [cstr, C.POINTER(Tuning)]),
     "spx_get_tuning": (C.c_int, [cstr, C.POINTER(Tuning)]),
@@ -137,6 +150,10 @@ SPXH_FUNCS = {
     "spxh_ogs_gather_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "spxh_ogs_gather_scatter_mixed": (C.c_int, [vp, C.c_int, C.c_int, vp]),
     "spxh_ogs_set_gs_format": (C.c_int, [vp, C.c_int, C.c_int]),
+    "spxh_ogs_setup_batch": (C.c_int, [vp, i32, C.POINTER(vp)]),
+    "spxh_ogs_autotune": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(GsTuning)]),
+    "spxh_cg_batch_autotune": (C.c_int, [i32, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.POINTER(vp),
+                                         C.POINTER(CgTuning)]),
     "spxh_srb_build": (C.c_int, [i32, vp, vp, i32, vp, i32, C.POINTER(vp)]),
     "spxh_srb_sizes": (C.c_int, [vp, vp]),
     "spxh_srb_copy": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -386,6 +386,9 @@ int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, v
   L.kf = (K >= 2 && K <= 4) ? K : 1;
   L.per = 0;
   L.variant = tu.variant;
+  if (srb->tile_lanes != 0 && srb->tile_lanes != 64 && srb->tile_lanes != 256)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile_lanes must be 0, 64 or 256");
+  if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
   L.stream = current_stream();
   L.err = kernel_error_flag();
   SrbArgs A;

@@ -164,6 +164,9 @@ int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, vo
   if ((rc = make_launch("spx_ogs_gather_scatter_srb", "ogs_gs_srb", srb->nblocks, K, srb->max_block_nnz, type, 1, L)))
     return rc;
   if (srb->xcd_remap >= 0) L.nxcd = srb->xcd_remap ? 8 : 1;
+  if (srb->tile_lanes != 0 && srb->tile_lanes != 64 && srb->tile_lanes != 256)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: tile_lanes must be 0, 64 or 256");
+  if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
   SrbArgs A;
   A.entryStarts = srb->entryStarts;
   A.chunkStarts = srb->chunkStarts;

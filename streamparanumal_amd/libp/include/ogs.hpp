@@ -74,6 +74,11 @@ class ogsBase_t {
                      const Method method, const bool _unique, const bool verbose,
                      platform_t &_platform);
   void Free();
+  // The block-diagonal union of independent set-up handles (no reference
+  // counterpart): local nodes of part i follow those of parts 0..i-1, its
+  // rows follow theirs, so one launch gathers every part.  All parts share
+  // one Kind; the total must fit dlong.
+  void SetupBatch(const std::vector<const ogsBase_t *> &parts, platform_t &_platform);
 
   // the local gather operator (CSR on the device); null before Setup
   std::shared_ptr<ogsOperator_t> localOperator() const { return gatherLocal; }

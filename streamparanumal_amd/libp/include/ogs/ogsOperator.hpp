@@ -23,7 +23,8 @@ constexpr int maxBlockNnz = 4096;
 // indices, with a link from each row member to the previous one.
 struct SortedRunBlocks {
   dlong nblocks = 0, cap = 0, nnz = 0, nchunks = 0, nruns = 0, nlinks = 0;
-  int xcdRemap = -1;  // launch hint passed in spx_srb_t (-1: library default)
+  int xcdRemap = -1;   // launch hints passed in spx_srb_t (-1 / 0: library default)
+  int tileLanes = 0;
   memory<dlong> entryStarts, chunkStarts, runBase, runIds, linkBase;
   memory<uint64_t> runMask, linkMask;
   memory<uint16_t> links;
@@ -34,6 +35,29 @@ struct SortedRunBlocks {
   void Upload(platform_t &platform);
   spx_srb_t Device() const;
 };
+
+// One launch shape of the Sym gather+scatter and its measured time.
+struct GsTuning {
+  int sorted;     // 1: sorted-run layout, 0: T-CSR blocks
+  int blockNnz;   // row block capacity
+  int tileLanes;  // sorted-run lanes per block (64 or 256)
+  int xcdRemap;   // sorted-run XCD remap (0 or 1)
+  double ms;
+};
+
+// One launch shape of the batched CG update (spx_cg_batch) and its time.
+struct CgTuning {
+  int targetBlocks;
+  int unroll;
+  int nontemporal;
+  double ms;
+};
+
+// Times spx_cg_batch for plans of several sizes and unroll / streaming
+// settings on the caller's vectors (alpha 0, so they keep their values);
+// leaves the best setting in the "cg_batch" tuning and its plan in *plan.
+CgTuning AutotuneCgBatch(const std::vector<int64_t> &segStarts, int storage, void *p, void *Ap, void *x, void *r,
+                         void *rdotr, int reps, spx_cg_plan_t *plan, std::vector<CgTuning> *log = nullptr);
 
 SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowStarts, const memory<dlong> colIds,
                                      const dlong Nblocks, const memory<dlong> blockStarts, const int cap);
@@ -104,6 +128,10 @@ class ogsOperator_t {
   // Sym Add gatherScatter on FP32 / BF16 storage with FP64 row sums
   // (storage: SPX_STORE_*)
   void GatherScatterMixedRaw(int storage, int k, void *v);
+  // Time every candidate launch shape of the Sym gather+scatter for values of
+  // `type` (storage < 0) or of mixed storage `storage`, k fields; keep and
+  // return the fastest.
+  GsTuning Autotune(int type, int storage, int k, int reps, std::vector<GsTuning> *log = nullptr);
 };
 
 // The local operator's CSR pair as built by the setup (host arrays).

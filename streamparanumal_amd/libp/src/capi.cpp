@@ -81,6 +81,22 @@ int spxh_ogs_setup(int32_t n, int64_t *ids, int kind, int unique, int verbose, s
   });
 }
 
+int spxh_ogs_setup_batch(const spxh_ogs_t *parts, int32_t nparts, spxh_ogs_t *out) {
+  if (!out || !parts || nparts < 1) return err(SPX_ERR_ARG, "spxh_ogs_setup_batch: bad arguments");
+  if (!state().open) return err(SPX_ERR_NODEVICE, "spxh_ogs_setup_batch: call spxh_platform_open first");
+  *out = nullptr;
+  return guarded([&] {
+    std::vector<const ogs::ogsBase_t *> v;
+    for (int32_t i = 0; i < nparts; ++i) {
+      LIBP_ABORT("spxh_ogs_setup_batch: null part " << i, !parts[i]);
+      v.push_back(&parts[i]->ogs);
+    }
+    auto h = std::make_unique<spxh_ogs_s>();
+    h->ogs.SetupBatch(v, state().platform);
+    *out = h.release();
+  });
+}
+
 int spxh_ogs_destroy(spxh_ogs_t h) {
   delete h;
   return SPX_OK;
@@ -165,6 +181,40 @@ int spxh_ogs_set_gs_format(spxh_ogs_t h, int sorted, int xcd_remap) {
     LIBP_ABORT("ogs not set up", !op);
     op->gsSorted = sorted != 0;
     op->gsSrb.xcdRemap = xcd_remap;
+  });
+}
+
+int spxh_ogs_autotune(spxh_ogs_t h, int type, int storage, int k, int reps, spxh_gs_tuning_t *best) {
+  if (!h || !best) return err(SPX_ERR_ARG, "spxh_ogs_autotune: null");
+  if (storage < -1 || storage > SPX_STORE_BF16) return err(SPX_ERR_ARG, "spxh_ogs_autotune: storage in -1..2");
+  if (storage < 0 && (type < 0 || type > 3)) return err(SPX_ERR_ARG, "spxh_ogs_autotune: unknown type");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    std::vector<ogs::GsTuning> log;
+    const ogs::GsTuning b = op->Autotune(type, storage, k, reps, &log);
+    best->sorted = b.sorted;
+    best->block_nnz = b.blockNnz;
+    best->tile_lanes = b.tileLanes;
+    best->xcd_remap = b.xcdRemap;
+    best->ms = b.ms;
+    best->candidates = (int32_t)log.size();
+  });
+}
+
+int spxh_cg_batch_autotune(int32_t nseg, const int64_t *segStarts, int storage, void *p, void *Ap, void *x, void *r,
+                           double *rdotr, int reps, spx_cg_plan_t *plan, spxh_cg_tuning_t *best) {
+  if (nseg < 1 || !segStarts || !plan || !best) return err(SPX_ERR_ARG, "spxh_cg_batch_autotune: bad arguments");
+  *plan = nullptr;
+  return guarded([&] {
+    std::vector<int64_t> st(segStarts, segStarts + nseg + 1);
+    std::vector<ogs::CgTuning> log;
+    const ogs::CgTuning b = ogs::AutotuneCgBatch(st, storage, p, Ap, x, r, rdotr, reps, plan, &log);
+    best->target_blocks = b.targetBlocks;
+    best->unroll = b.unroll;
+    best->nontemporal = b.nontemporal;
+    best->ms = b.ms;
+    best->candidates = (int32_t)log.size();
   });
 }
 

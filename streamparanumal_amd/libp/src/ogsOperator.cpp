@@ -71,9 +71,10 @@ void ogsOperator_t::SetGsBlockNnz(int nnz) {
   gsBlockNnz = nnz;
   blockRowsGreedy(NrowsT, rowStartsT, gsBlockNnz, NgsBlocks, gsBlockRowStarts);
   o_gsBlockRowStarts = platform.malloc<dlong>(NgsBlocks + 1, gsBlockRowStarts);
-  const int xcd = gsSrb.xcdRemap;
+  const int xcd = gsSrb.xcdRemap, lanes = gsSrb.tileLanes;
   gsSrb = BuildSortedRunBlocks(NrowsT, rowStartsT, colIdsT, NgsBlocks, gsBlockRowStarts, gsBlockNnz);
   gsSrb.xcdRemap = xcd;
+  gsSrb.tileLanes = lanes;
   gsSrb.Upload(platform);
 }
 

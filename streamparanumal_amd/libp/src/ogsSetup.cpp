@@ -18,6 +18,7 @@
 // (ogsSetup.cpp:347), which the C++ standard does not fix.
 #include <algorithm>
 #include <chrono>
+#include <limits>
 #include <cstdlib>
 #include <numeric>
 
@@ -233,6 +234,55 @@ void ogsBase_t::Setup(const dlong _N, memory<hlong> ids, comm_t _comm, const Kin
   if (verbose && comm.rank() == 0)
     std::cout << "ogs Setup: " << NlocalT << " gather nodes, " << gatherLocal->nnzT << " entries; setup "
               << setupSeconds_ << " seconds." << std::endl;
+}
+
+void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform_t &_platform) {
+  Free();
+  const auto t0 = std::chrono::steady_clock::now();
+  LIBP_ABORT("SetupBatch needs at least one part", parts.empty());
+  platform = _platform;
+  comm = comm_t::world();
+  kind = parts[0]->kind;
+  unique = parts[0]->unique;
+  gather_defined = true;
+  long long n = 0, rowsT = 0, rowsN = 0, nnzT = 0, nnzN = 0;
+  for (const ogsBase_t *p : parts) {
+    LIBP_ABORT("SetupBatch: a part is not set up", !p || !p->gatherLocal);
+    LIBP_ABORT("SetupBatch: parts must share one Kind", p->kind != kind);
+    n += p->N;
+    rowsT += p->gatherLocal->NrowsT;
+    rowsN += p->gatherLocal->NrowsN;
+    nnzT += p->gatherLocal->nnzT;
+    nnzN += p->gatherLocal->nnzN;
+    gather_defined = gather_defined && p->gather_defined;
+  }
+  LIBP_ABORT("SetupBatch: " << n << " local nodes in all exceed dlong", n > std::numeric_limits<dlong>::max());
+  N = (dlong)n;
+  memory<dlong> rsT(rowsT + 1), ciT(nnzT > 0 ? nnzT : 1), rsN(rowsN + 1), ciN(nnzN > 0 ? nnzN : 1);
+  dlong nodeOff = 0, rowOffT = 0, rowOffN = 0, eOffT = 0, eOffN = 0;
+  rsT[0] = 0;
+  rsN[0] = 0;
+  for (const ogsBase_t *p : parts) {
+    const ogsOperator_t &op = *p->gatherLocal;
+    for (dlong r = 0; r < op.NrowsT; ++r) rsT[rowOffT + r + 1] = eOffT + op.rowStartsT[r + 1];
+    for (dlong e = 0; e < op.nnzT; ++e) ciT[eOffT + e] = nodeOff + op.colIdsT[e];
+    for (dlong r = 0; r < op.NrowsN; ++r) rsN[rowOffN + r + 1] = eOffN + op.rowStartsN[r + 1];
+    for (dlong e = 0; e < op.nnzN; ++e) ciN[eOffN + e] = nodeOff + op.colIdsN[e];
+    nodeOff += p->N;
+    rowOffT += op.NrowsT;
+    rowOffN += op.NrowsN;
+    eOffT += op.nnzT;
+    eOffN += op.nnzN;
+  }
+  NlocalT = (dlong)rowsT;
+  NlocalP = (dlong)rowsN;
+  NhaloT = 0;
+  NhaloP = 0;
+  Ngather = NlocalP;
+  NgatherGlobal = Ngather;
+  gatherLocal = std::make_shared<ogsOperator_t>(platform, kind == Unsigned ? Unsigned : Signed, NlocalP, NlocalT, N,
+                                                rsT, ciT, rsN, ciN);
+  setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void ogsBase_t::Free() {

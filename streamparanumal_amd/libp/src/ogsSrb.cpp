@@ -129,6 +129,7 @@ spx_srb_t SortedRunBlocks::Device() const {
   d.nblocks = nblocks;
   d.max_block_nnz = cap;
   d.xcd_remap = xcdRemap;
+  d.tile_lanes = tileLanes;
   d.entryStarts = o_entryStarts.ptr();
   d.chunkStarts = o_chunkStarts.ptr();
   d.runMask = o_runMask.ptr();

@@ -86,10 +86,10 @@ def srb_build(row_starts, col_ids, block_starts, cap):
 class SrbDevice:
     """A sorted-run layout uploaded to the device; `.desc` is the spx_srb_t to pass by reference."""
 
-    def __init__(self, srb, xcd_remap=-1):
+    def __init__(self, srb, xcd_remap=-1, tile_lanes=0):
         from .device import DeviceArray
         self.arrays = {k: DeviceArray.from_numpy(srb[k]) for k in SRB_KEYS}
-        self.desc = _lib.Srb(srb["nblocks"], srb["max_block_nnz"], xcd_remap,
+        self.desc = _lib.Srb(srb["nblocks"], srb["max_block_nnz"], xcd_remap, tile_lanes,
                              *(self.arrays[k].ptr for k in SRB_KEYS))
 
     def gather_scatter(self, q, k=1, op=_lib.ADD):
@@ -119,6 +119,17 @@ class Ogs:
                                            1 if verbose else 0, C.byref(self.h)), "spxh_ogs_setup")
         self.ids = ids  # sign-flagged when unique
         self.refresh()
+
+    @classmethod
+    def batch(cls, parts):
+        """The block-diagonal union of set-up handles: part i's local nodes follow parts 0..i-1's."""
+        self = cls.__new__(cls)
+        arr = (C.c_void_p * len(parts))(*[p.h.value for p in parts])
+        self.h = C.c_void_p()
+        checkh(libspxhost().spxh_ogs_setup_batch(arr, len(parts), C.byref(self.h)), "spxh_ogs_setup_batch")
+        self.ids = None
+        self.refresh()
+        return self
 
     def refresh(self):
         info = _lib.OgsInfo()
@@ -151,6 +162,18 @@ class Ogs:
         xcd_remap: sorted-run launch hint (-1 library default)."""
         checkh(libspxhost().spxh_ogs_set_gs_format(self.h, 1 if sorted_runs else 0, xcd_remap),
                "spxh_ogs_set_gs_format")
+
+    def autotune(self, dtype=np.float64, k=1, reps=5, mixed=False):
+        """Time every launch shape of the Sym gather+scatter for k fields of dtype (mixed=True: the
+        FP64-sum kernel on float32 / uint16-bf16 storage), keep the fastest, return it."""
+        best = _lib.GsTuning()
+        if mixed:
+            rc = libspxhost().spxh_ogs_autotune(self.h, 0, _lib.storage_of(dtype), k, reps, C.byref(best))
+        else:
+            rc = libspxhost().spxh_ogs_autotune(self.h, ogs_type(np.dtype(dtype)), -1, k, reps, C.byref(best))
+        checkh(rc, "spxh_ogs_autotune")
+        self.refresh()
+        return {f: getattr(best, f) for f, _ in _lib.GsTuning._fields_}
 
     def set_scatter_path(self, by_rows):
         """Scatter over the CSR row blocks (True, default) or through the local->row map (False)."""

@@ -2,7 +2,7 @@
 import numpy as np
 
 from . import _lib
-from ._lib import check, libspx
+from ._lib import check, checkh, libspx, libspxhost
 from .device import DeviceArray
 
 
@@ -82,3 +82,50 @@ class MixedCG:
                 raise TypeError("p, Ap, x, r must share one storage type")
         check(libspx().spx_bs5_full_mixed(st, n, alpha, p.ptr, Ap.ptr, x.ptr, r.ptr, self.tmp.ptr,
                                           self.out.ptr), "spx_bs5_full_mixed")
+
+
+class CgPlan:
+    """Batched CG update over segments of back-to-back vectors (spx_cg_batch): x += alpha p,
+    r -= alpha Ap per segment, rdotr[s] in FP64.  seg_starts: nseg+1 entry offsets, each a
+    multiple of 4."""
+
+    def __init__(self, seg_starts, target_blocks=1024):
+        import ctypes as C
+        st = np.ascontiguousarray(seg_starts, dtype=np.int64)
+        self.nseg = st.size - 1
+        self.h = C.c_void_p()
+        check(libspx().spx_cg_plan_create(self.nseg, st.ctypes.data, target_blocks, C.byref(self.h)),
+              "spx_cg_plan_create")
+        self.nblocks = libspx().spx_cg_plan_blocks(self.h)
+        self.rdotr = DeviceArray(self.nseg)
+
+    @classmethod
+    def autotuned(cls, seg_starts, p, Ap, x, r, reps=5):
+        """Plan and "cg_batch" tuning picked by timing every candidate on these vectors (values kept)."""
+        import ctypes as C
+        self = cls.__new__(cls)
+        st = np.ascontiguousarray(seg_starts, dtype=np.int64)
+        self.nseg = st.size - 1
+        self.rdotr = DeviceArray(self.nseg)
+        self.h = C.c_void_p()
+        best = _lib.CgTuning()
+        checkh(libspxhost().spxh_cg_batch_autotune(self.nseg, st.ctypes.data, _lib.storage_of(p.dtype), p.ptr, Ap.ptr,
+                                                   x.ptr, r.ptr, self.rdotr.ptr, reps, C.byref(self.h),
+                                                   C.byref(best)), "spxh_cg_batch_autotune")
+        self.nblocks = libspx().spx_cg_plan_blocks(self.h)
+        self.tuning = {f: getattr(best, f) for f, _ in _lib.CgTuning._fields_}
+        return self
+
+    def update(self, alpha, p, Ap, x, r):
+        st = _lib.storage_of(p.dtype)
+        check(libspx().spx_cg_batch(st, self.h, alpha, p.ptr, Ap.ptr, x.ptr, r.ptr, self.rdotr.ptr), "spx_cg_batch")
+
+    def values(self):
+        return self.rdotr.to_numpy()
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                libspx().spx_cg_plan_destroy(self.h)
+        except Exception:
+            pass

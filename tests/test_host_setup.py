@@ -152,3 +152,22 @@ def test_srb_rejects_bad_blocks():
         ogs.srb_build(rs, np.array([0, 0, 2], np.int32), np.array([0, 2], np.int32), 4)  # node twice
     empty = ogs.srb_build(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(1, np.int32), 16)
     assert empty["nblocks"] == 0 and empty["nnz"] == 0
+
+
+# ---------------------------------------------------------------------------
+# batched instances: the generator is the reference LCG stream
+# ---------------------------------------------------------------------------
+
+def test_batch_instances_follow_the_lcg():
+    from streamparanumal_amd.batch import instances, lcg_uniform01
+    u = np.array(lcg_uniform01(200, 4))
+    assert np.array_equal(u, (o.uniform(200, 4) + 1.0) / 2.0)     # U[-1,1) draw -> U[0,1), exactly
+    inst = instances(64, 4)
+    assert len(inst) == 64
+    assert all(4 <= n <= 48 and p in (3, 5, 7, 9) for n, p in inst)
+    assert inst[:4] == [(4 + int(u[0] * 45), (3, 5, 7, 9)[int(u[1] * 4)]),
+                        (4 + int(u[2] * 45), (3, 5, 7, 9)[int(u[3] * 4)]),
+                        (4 + int(u[4] * 45), (3, 5, 7, 9)[int(u[5] * 4)]),
+                        (4 + int(u[6] * 45), (3, 5, 7, 9)[int(u[7] * 4)])]
+    # every degree and both ends of the range occur over the 64 draws
+    assert {p for _, p in inst} == {3, 5, 7, 9}
