@@ -155,6 +155,41 @@ def batched_suite(K, W, count=64, seed=4, fields=3):
     return out
 
 
+def degree_sweep(K, W, seed=2, target=1e8):
+    """FP64 gather (Trans), scatter (NoTrans) and gather+scatter (Sym) on hex boxes of
+    round((target/(p+1)^3)^(1/3)) elements per side for p = 1..15 (about `target` local nodes
+    each), U[-1,1) values from `seed`; library-default launch shapes."""
+    import streamparanumal_amd as spx
+    from streamparanumal_amd import device, ogs
+    from streamparanumal_amd.device import DeviceArray
+
+    rows = []
+    for p in range(1, 16):
+        n = int(round((target / (p + 1) ** 3) ** (1.0 / 3.0)))
+        ids = ogs.hex_ids(n, n, n, p)
+        g = ogs.Ogs(ids)
+        del ids
+        Nl, Ng = g.N, g.NgatherGlobal
+        q = DeviceArray(Nl)
+        device.fill_uniform(q, seed, 0)
+        gq = DeviceArray(Ng)
+        b6 = 4 * (Ng + 1) + 12 * Nl + 8 * Ng
+        b7 = 4 * (Ng + 1) + 4 * Nl + 8 * Ng + 8 * Nl
+        b8 = 4 * (Ng + 1) + 4 * Nl + 16 * Nl
+        _, e6 = timed(lambda: g.gather(gq, q, 1, spx.ADD, spx.TRANS), K, W)
+        _, e7 = timed(lambda: g.scatter(q, gq, 1, spx.NOTRANS), K, W)
+        _, e8 = timed(lambda: g.gather_scatter(q), K, W)
+        device.check_kernel_errors()
+        row = {"p": p, "n": n, "local_nodes": Nl, "global_nodes": Ng, "Ng_over_Nl": round(Ng / Nl, 3),
+               "gather_GBs": round(b6 / e6 / 1e9, 1), "scatter_GBs": round(b7 / e7 / 1e9, 1),
+               "gather_scatter_GBs": round(b8 / e8 / 1e9, 1), "gather_GDOFs": round(Ng / e6 / 1e9, 2),
+               "scatter_GDOFs": round(Ng / e7 / 1e9, 2), "gather_scatter_GDOFs": round(Ng / e8 / 1e9, 2)}
+        rows.append(row)
+        log("degree sweep", json.dumps(row))
+        del g, q, gq
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +200,7 @@ def main():
     ap.add_argument("--log2n", type=int, default=27, help="BS1-BS5 vector length 2^log2n")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-mesh batched comparison")
+    ap.add_argument("--no-degree-sweep", action="store_true", help="skip the p=1..15 sweep at ~1e8 nodes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -329,6 +365,10 @@ def main():
     suite["BS0_launch"] = {"launches": nl, "plain_us_per_launch": round(plain * 1e6, 3),
                            "graph_us_per_launch": round(replay * 1e6, 3)}
     log("BS0", json.dumps(suite["BS0_launch"]))
+
+    # ------------------------------------------------------------ degree sweep
+    if not args.no_degree_sweep:
+        suite["degree_sweep_1e8"] = degree_sweep(K, W)
 
     # ------------------------------------------------------- batched instances
     if not args.no_batched:

@@ -231,10 +231,18 @@ typedef struct {
   const uint64_t *linkMask;
   const int32_t *linkBase;
   const uint16_t *links;
+  const int32_t *chunkRow; /* CSR row of the first row-leading position at or after each chunk's start
+                              (gather / scatter on the layout; may be NULL for gather+scatter) */
 } spx_srb_t;
 /* gatherScatter, ogs::Sym (ogsKernels.okl:32-81), on the sorted-run layout:
    same result as spx_ogs_gather_scatter on the CSR it was built from */
 int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, void *q);
+/* gather (ogsKernels.okl:86-122) and scatter (ogsKernels.okl:127-164) on the
+   sorted-run layout of a CSR: same results as spx_ogs_gather /
+   spx_ogs_scatter on that CSR; the row of a position is chunkRow[c] plus the
+   row-leading positions before it in its chunk */
+int spx_ogs_gather_srb(int type, int op, int K, const spx_srb_t *srb, const void *v, void *gv);
+int spx_ogs_scatter_srb(int type, int K, const spx_srb_t *srb, const void *gv, void *v);
 
 /* ------------------------------------------- reduced-precision storage -- */
 /* Storage types of the mixed-precision variants (no reference counterpart:

@@ -70,7 +70,7 @@ class Srb(C.Structure):
     """spx_srb_t: device addresses of a sorted-run row block layout."""
     _fields_ = [("nblocks", i32), ("max_block_nnz", i32), ("xcd_remap", i32), ("tile_lanes", i32), ("entryStarts", vp),
                 ("chunkStarts", vp), ("runMask", vp), ("runBase", vp), ("runIds", vp), ("linkMask", vp),
-                ("linkBase", vp), ("links", vp)]
+                ("linkBase", vp), ("links", vp), ("chunkRow", vp)]
 
 
 # name: (restype, argtypes) -- every function of include/spx.h
@@ -125,6 +125,8 @@ SPX_FUNCS = {
     "spx_ogs_extract": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp]),
     "spx_ogs_gather_scatter_srb": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Srb), vp]),
     "spx_ogs_gather_scatter_srb_mixed": (C.c_int, [C.c_int, C.c_int, C.POINTER(Srb), vp]),
+    "spx_ogs_gather_srb": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Srb), vp, vp]),
+    "spx_ogs_scatter_srb": (C.c_int, [C.c_int, C.c_int, C.POINTER(Srb), vp, vp]),
     "spx_ogs_gather_scatter_mixed": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, i32, vp]),
     "spx_cg_plan_create": (C.c_int, [i32, vp, i32, C.POINTER(vp)]),
     "spx_cg_plan_destroy": (C.c_int, [vp]),
@@ -151,12 +153,13 @@ SPXH_FUNCS = {
     "spxh_ogs_gather_scatter_mixed": (C.c_int, [vp, C.c_int, C.c_int, vp]),
     "spxh_ogs_set_gs_format": (C.c_int, [vp, C.c_int, C.c_int]),
     "spxh_ogs_setup_batch": (C.c_int, [vp, i32, C.POINTER(vp)]),
+    "spxh_ogs_set_gather_format": (C.c_int, [vp, C.c_int, C.c_int]),
     "spxh_ogs_autotune": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(GsTuning)]),
     "spxh_cg_batch_autotune": (C.c_int, [i32, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.POINTER(vp),
                                          C.POINTER(CgTuning)]),
     "spxh_srb_build": (C.c_int, [i32, vp, vp, i32, vp, i32, C.POINTER(vp)]),
     "spxh_srb_sizes": (C.c_int, [vp, vp]),
-    "spxh_srb_copy": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "spxh_srb_copy": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "spxh_srb_destroy": (C.c_int, [vp]),
     "spxh_ogs_gather": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_ogs_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp]),

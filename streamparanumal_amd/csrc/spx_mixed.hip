@@ -91,7 +91,9 @@ int launch_gs_mixed(const Launch &L, const int *bs, const int *st, const int *id
   else if (L.kf == 2 && L.per == 4) SPX_GSM(2, 4);
   else if (L.kf == 2 && L.per == 8) SPX_GSM(2, 8);
   else if (L.kf == 3 && L.per == 4) SPX_GSM(3, 4);
+  else if (L.kf == 3 && L.per == 8) SPX_GSM(3, 8);
   else if (L.kf == 4 && L.per == 4) SPX_GSM(4, 4);
+  else if (L.kf == 4 && L.per == 8) SPX_GSM(4, 8);
   else return fail(SPX_ERR_UNSUPPORTED, "spx_ogs_gather_scatter_mixed: no kernel for this tile");
 #undef SPX_GSM
   return SPX_OK;
@@ -135,7 +137,9 @@ int launch_gs_srb_mixed(const Launch &L, const SrbArgs &A, S *q) {
   else if (W.kf == 2 && W.per == 4) SPX_GSRM(2, 4);
   else if (W.kf == 2 && W.per == 8) SPX_GSRM(2, 8);
   else if (W.kf == 3 && W.per == 4) SPX_GSRM(3, 4);
+  else if (W.kf == 3 && W.per == 8) SPX_GSRM(3, 8);
   else if (W.kf == 4 && W.per == 4) SPX_GSRM(4, 4);
+  else if (W.kf == 4 && W.per == 8) SPX_GSRM(4, 8);
   else return fail(SPX_ERR_UNSUPPORTED, "spx_ogs_gather_scatter_srb_mixed: no kernel for this tile");
 #undef SPX_GSRM
   return SPX_OK;

@@ -185,6 +185,66 @@ int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, vo
   return SPX_OK;
 }
 
+namespace {
+int srb_launch_setup(const char *name, int type, int K, const spx_srb_t *srb, Launch &L, SrbArgs &A) {
+  if (!srb) return fail(SPX_ERR_ARG, std::string(name) + ": null layout");
+  int rc = check_common(name, type, K, srb->nblocks, srb->max_block_nnz);
+  if (rc) return rc;
+  if (srb->nblocks == 0) return SPX_OK;
+  if (!srb->entryStarts || !srb->chunkStarts || !srb->runMask || !srb->runBase || !srb->runIds || !srb->linkMask ||
+      !srb->linkBase || !srb->links || !srb->chunkRow)
+    return fail(SPX_ERR_ARG, std::string(name) + ": null layout array (chunkRow is needed)");
+  if ((rc = make_launch(name, "ogs_gs_srb", srb->nblocks, K, srb->max_block_nnz, type, 1, L))) return rc;
+  if (srb->xcd_remap >= 0) L.nxcd = srb->xcd_remap ? 8 : 1;
+  if (srb->tile_lanes != 0 && srb->tile_lanes != 64 && srb->tile_lanes != 256)
+    return fail(SPX_ERR_ARG, std::string(name) + ": tile_lanes must be 0, 64 or 256");
+  if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
+  A.entryStarts = srb->entryStarts;
+  A.chunkStarts = srb->chunkStarts;
+  A.runMask = reinterpret_cast<const unsigned long long *>(srb->runMask);
+  A.runBase = srb->runBase;
+  A.runIds = srb->runIds;
+  A.linkMask = reinterpret_cast<const unsigned long long *>(srb->linkMask);
+  A.linkBase = srb->linkBase;
+  A.links = srb->links;
+  return SPX_OK;
+}
+}  // namespace
+
+int spx_ogs_gather_srb(int type, int op, int K, const spx_srb_t *srb, const void *v, void *gv) {
+  SPX_NEED_DEVICE();
+  Launch L;
+  SrbArgs A;
+  int rc = srb_launch_setup("spx_ogs_gather_srb", type, K, srb, L, A);
+  if (rc || (rc = check_op("spx_ogs_gather_srb", op))) return rc;
+  if (srb->nblocks == 0) return SPX_OK;
+  if (!v || !gv) return fail(SPX_ERR_ARG, "spx_ogs_gather_srb: null pointer");
+  rc = by_type(type, [&](auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return launch_gather_srb(L, op, A, srb->chunkRow, (const T *)v, (T *)gv);
+  });
+  if (rc) return rc;
+  SPX_LAUNCHED("spx_ogs_gather_srb");
+  return SPX_OK;
+}
+
+int spx_ogs_scatter_srb(int type, int K, const spx_srb_t *srb, const void *gv, void *v) {
+  SPX_NEED_DEVICE();
+  Launch L;
+  SrbArgs A;
+  int rc = srb_launch_setup("spx_ogs_scatter_srb", type, K, srb, L, A);
+  if (rc) return rc;
+  if (srb->nblocks == 0) return SPX_OK;
+  if (!v || !gv) return fail(SPX_ERR_ARG, "spx_ogs_scatter_srb: null pointer");
+  rc = by_type(type, [&](auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return launch_scatter_srb(L, A, srb->chunkRow, (const T *)gv, (T *)v);
+  });
+  if (rc) return rc;
+  SPX_LAUNCHED("spx_ogs_scatter_srb");
+  return SPX_OK;
+}
+
 int spx_ogs_scatter_inverse(int type, int K, int32_t n, const int32_t *g2l, const void *gv, void *v) {
   SPX_NEED_DEVICE();
   if (!type_size(type)) return fail(SPX_ERR_ARG, "spx_ogs_scatter_inverse: unknown type");

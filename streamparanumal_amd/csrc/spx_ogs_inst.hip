@@ -22,7 +22,9 @@ static int dispatch(int op, int kf, int per, F &&f) {
     if (kf == 2 && per == 4) return f.template operator()<OPC, 2, 4>();
     if (kf == 2 && per == 8) return f.template operator()<OPC, 2, 8>();
     if (kf == 3 && per == 4) return f.template operator()<OPC, 3, 4>();
+    if (kf == 3 && per == 8) return f.template operator()<OPC, 3, 8>();
     if (kf == 4 && per == 4) return f.template operator()<OPC, 4, 4>();
+    if (kf == 4 && per == 8) return f.template operator()<OPC, 4, 8>();
     return fail(SPX_ERR_UNSUPPORTED, "ogs: no kernel for fields-per-pass " + std::to_string(kf) +
                                          " with " + std::to_string(per) + " entries per lane");
   };
@@ -77,6 +79,49 @@ int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q) {
   }
   return dispatch<true>(op, L.kf, L.per, [&]<int OPC, int KF, int PER>() {
     hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER>), grid_of(L), dim3(kThreads), lds, L.stream, L, SRB_ARGS(A), q);
+    return SPX_OK;
+  });
+}
+
+// sorted-run gather / scatter: one-wave tiles (variant 2, cap <= 1024) or 256 lanes
+template <bool WITH_OP, typename F>
+static int srb_shape(const Launch &L, int op, F &&f) {
+  Launch W = L;
+  const bool wave = L.variant == 2 && L.cap <= 16 * 64;
+  const int lanes = wave ? 64 : kThreads;
+  W.per = pick_per(W.kf, W.cap * (kThreads / lanes));
+  if (!W.per) {
+    W.kf = 1;
+    W.per = pick_per(1, W.cap * (kThreads / lanes));
+  }
+  if (!W.per) return fail(SPX_ERR_ARG, "ogs sorted-run: tile too large");
+  const size_t lds = lds_bytes(W.cap, W.kf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short);
+  if (lds > 65536) return fail(SPX_ERR_ARG, "ogs sorted-run: tile too large for LDS");
+  return dispatch<WITH_OP>(op, W.kf, W.per, [&]<int OPC, int KF, int PER>() {
+    return f.template operator()<OPC, KF, PER>(W, wave, lds);
+  });
+}
+
+int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const int *chunkRow, const T *v, T *gv) {
+  return srb_shape<true>(L, op, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
+    if (wave)
+      hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A),
+                         (T *)nullptr, chunkRow, v, gv);
+    else
+      hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A),
+                         (T *)nullptr, chunkRow, v, gv);
+    return SPX_OK;
+  });
+}
+
+int launch_scatter_srb(const Launch &L, const SrbArgs &A, const int *chunkRow, const T *gv, T *v) {
+  return srb_shape<false>(L, SPX_ADD, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
+    if (wave)
+      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A),
+                         (T *)nullptr, chunkRow, gv, v);
+    else
+      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A),
+                         (T *)nullptr, chunkRow, gv, v);
     return SPX_OK;
   });
 }

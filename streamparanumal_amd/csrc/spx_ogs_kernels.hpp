@@ -326,6 +326,136 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS) {
       for (int f = 0; f < KF; ++f) q[(int64_t)id[j] * K + kb + f] = vals[(t + j * NT) * KF + f];
 }
 
+// Gather and scatter on the sorted-run layout.  A position's row is
+// chunkRow[c] plus the row-leading (unlinked) positions before it in its
+// chunk; row-leading positions hold the rows in CSR order.
+template <int PER, int NT>
+__device__ __forceinline__ void srb_decode(const SrbArgs &A, const int *__restrict__ chunkRow, int c0, int nch,
+                                           int nnz, int (&id)[PER], int (&prev)[PER], int (&row)[PER]) {
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int c = c0 + min((NT / 64) * j + w, nch - 1);  // wave-uniform
+    const unsigned long long rm = A.runMask[c], lm = A.linkMask[c];
+    const int rbase = A.runBase[c], lbase = A.linkBase[c], crow = chunkRow[c];
+    const int l = clampi(t + j * NT, nnz - 1) & 63;
+    const unsigned long long upto = (2ull << l) - 1ull;
+    const unsigned long long rbel = rm & upto;
+    id[j] = A.runIds[rbase + __builtin_popcountll(rbel) - 1] + (l - (63 - __builtin_clzll(rbel)));
+    const int linkedBefore = __builtin_popcountll(lm & (upto >> 1));
+    const bool linked = (lm >> l) & 1ull;
+    prev[j] = linked ? (int)A.links[lbase + linkedBefore] : -1;
+    row[j] = crow + (l - linkedBefore);  // valid for row-leading positions
+  }
+}
+
+template <typename T, int OP, int KF, int PER, int NT = kThreads>
+__global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const int *__restrict__ chunkRow,
+                                                   const T *__restrict__ v, T *__restrict__ gv) {
+  const SrbArgs A{entryStarts, chunkStarts, runMask, linkMask, runBase, runIds, linkBase, links};
+  (void)q;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *vals = reinterpret_cast<T *>(lds_raw);
+  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int kb = blockIdx.y * KF;
+  const int e0 = A.entryStarts[b];
+  const int nnz = A.entryStarts[b + 1] - e0;
+  const int c0 = A.chunkStarts[b];
+  const int nch = A.chunkStarts[b + 1] - c0;
+  if (nnz > L.cap || nch != (nnz + 63) / 64) {
+    flag_error(L.err, 7);
+    return;
+  }
+  if (nnz == 0) return;
+  const int64_t K = L.K;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz) next[t + j * NT] = kEnd;
+  int id[PER], prev[PER], row[PER];
+  srb_decode<PER, NT>(A, chunkRow, c0, nch, nnz, id, prev, row);
+  T x[PER][KF];
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+#pragma unroll
+    for (int f = 0; f < KF; ++f) x[j][f] = v[(int64_t)id[j] * K + kb + f];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz) {
+#pragma unroll
+      for (int f = 0; f < KF; ++f) vals[(t + j * NT) * KF + f] = x[j][f];
+      if (prev[j] >= 0 && prev[j] < nnz) next[prev[j]] = (unsigned short)(t + j * NT);
+    }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz && prev[j] < 0) {
+#pragma unroll
+      for (int f = 0; f < KF; ++f) {
+        T acc = Fold<T, OP>::init();
+        for (int i = t + j * NT, s = 0; i < nnz && s < nnz; i = next[i], ++s) Fold<T, OP>::apply(acc, vals[i * KF + f]);
+        gv[(int64_t)row[j] * K + kb + f] = acc;
+      }
+    }
+}
+
+template <typename T, int KF, int PER, int NT = kThreads>
+__global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const int *__restrict__ chunkRow,
+                                                    const T *__restrict__ gv, T *__restrict__ v) {
+  const SrbArgs A{entryStarts, chunkStarts, runMask, linkMask, runBase, runIds, linkBase, links};
+  (void)q;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *vals = reinterpret_cast<T *>(lds_raw);
+  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int kb = blockIdx.y * KF;
+  const int e0 = A.entryStarts[b];
+  const int nnz = A.entryStarts[b + 1] - e0;
+  const int c0 = A.chunkStarts[b];
+  const int nch = A.chunkStarts[b + 1] - c0;
+  if (nnz > L.cap || nch != (nnz + 63) / 64) {
+    flag_error(L.err, 8);
+    return;
+  }
+  if (nnz == 0) return;
+  const int64_t K = L.K;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz) next[t + j * NT] = kEnd;
+  int id[PER], prev[PER], row[PER];
+  srb_decode<PER, NT>(A, chunkRow, c0, nch, nnz, id, prev, row);
+  // every lane reads the value of its position's row when it leads the row
+  // (a clamped row otherwise: an in-bounds load whose value is dropped)
+  T g[PER][KF];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int64_t r = prev[j] < 0 ? row[j] : (int64_t)chunkRow[c0];
+#pragma unroll
+    for (int f = 0; f < KF; ++f) g[j][f] = gv[r * K + kb + f];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz && prev[j] >= 0 && prev[j] < nnz) next[prev[j]] = (unsigned short)(t + j * NT);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz && prev[j] < 0)
+#pragma unroll
+      for (int f = 0; f < KF; ++f)
+        for (int i = t + j * NT, s = 0; i < nnz && s < nnz; i = next[i], ++s) vals[i * KF + f] = g[j][f];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t + j * NT < nnz)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) v[(int64_t)id[j] * K + kb + f] = vals[(t + j * NT) * KF + f];
+}
+
 // ------------------------------------------------------------ general GS --
 // gather CSR (gStarts/gIds, ngrows rows) and scatter CSR (sStarts/sIds,
 // nsrows rows) differ (ogs::NoTrans, ogs::Trans).  Rows at or past ngrows
@@ -555,7 +685,7 @@ __global__ __launch_bounds__(kThreads) void k_extract(int64_t total, int K, cons
 inline int pick_per(int kf, int cap) {
   const int need = (cap + kThreads - 1) / kThreads;
   if (need <= 4) return 4;
-  if (need <= 8 && kf <= 2) return 8;
+  if (need <= 8 && kf <= 4) return 8;
   if (need <= 16 && kf == 1) return 16;
   return 0;
 }
@@ -580,7 +710,9 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
   int launch_scatter(const Launch &L, const int *bs, const int *st, const int *ids, const T *gv, T *v);    \
   int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq);           \
   int launch_scatter_inv(hipStream_t s, int64_t n, int K, const int *g2l, const T *gv, T *v, int npl); \
-  int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q);
+  int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q);                                       \
+  int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const int *chunkRow, const T *v, T *gv); \
+  int launch_scatter_srb(const Launch &L, const SrbArgs &A, const int *chunkRow, const T *gv, T *v);
 SPX_OGS_GEN_DECL(float)
 SPX_OGS_GEN_DECL(double)
 SPX_OGS_GEN_DECL(int)

@@ -218,6 +218,16 @@ int spxh_cg_batch_autotune(int32_t nseg, const int64_t *segStarts, int storage, 
   });
 }
 
+int spxh_ogs_set_gather_format(spxh_ogs_t h, int gather_sorted, int scatter_sorted) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_gather_format: null");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    if (gather_sorted >= 0) op->gatherSorted = gather_sorted != 0;
+    if (scatter_sorted >= 0) op->scatterSorted = scatter_sorted != 0;
+  });
+}
+
 int spxh_ogs_global_to_local(spxh_ogs_t h, int32_t *g2l) {
   if (!h || !g2l) return err(SPX_ERR_ARG, "spxh_ogs_global_to_local: null");
   return guarded([&] {
@@ -366,7 +376,7 @@ int spxh_srb_sizes(spxh_srb_t h, int64_t *sizes) {
 }
 
 int spxh_srb_copy(spxh_srb_t h, int32_t *entryStarts, int32_t *chunkStarts, uint64_t *runMask, int32_t *runBase,
-                  int32_t *runIds, uint64_t *linkMask, int32_t *linkBase, uint16_t *links) {
+                  int32_t *runIds, uint64_t *linkMask, int32_t *linkBase, uint16_t *links, int32_t *chunkRow) {
   if (!h) return err(SPX_ERR_ARG, "spxh_srb_copy: null");
   const ogs::SortedRunBlocks &s = h->srb;
   auto cp = [](const auto &m, size_t n, auto *dst) {
@@ -380,6 +390,7 @@ int spxh_srb_copy(spxh_srb_t h, int32_t *entryStarts, int32_t *chunkStarts, uint
   cp(s.linkMask, (size_t)s.nchunks, linkMask);
   cp(s.linkBase, (size_t)s.nchunks, linkBase);
   cp(s.links, (size_t)s.nlinks + 1, links);
+  cp(s.chunkRow, (size_t)s.nchunks, chunkRow);
   return SPX_OK;
 }
 

@@ -109,6 +109,9 @@ void ogsOperator_t::GatherRaw(Type type, Op op, Transpose trans, int k, const vo
     if (NrowBlocksN)
       SPX_CHECK(spx_ogs_gather((int)type, (int)op, k, NrowBlocksN, o_blockRowStartsN.ptr(), o_rowStartsN.ptr(),
                                o_colIdsN.ptr(), blockNnz, v, gv));
+  } else if (gatherSorted && NgsBlocks) {
+    const spx_srb_t d = gsSrb.Device();
+    SPX_CHECK(spx_ogs_gather_srb((int)type, (int)op, k, &d, v, gv));
   } else {
     if (NrowBlocksT)
       SPX_CHECK(spx_ogs_gather((int)type, (int)op, k, NrowBlocksT, o_blockRowStartsT.ptr(), o_rowStartsT.ptr(),
@@ -127,6 +130,9 @@ void ogsOperator_t::ScatterRaw(Type type, Transpose trans, int k, const void *gv
     if (NrowBlocksN)
       SPX_CHECK(spx_ogs_scatter((int)type, k, NrowBlocksN, o_blockRowStartsN.ptr(), o_rowStartsN.ptr(),
                                 o_colIdsN.ptr(), blockNnz, gv, v));
+  } else if (scatterSorted && NgsBlocks) {
+    const spx_srb_t d = gsSrb.Device();
+    SPX_CHECK(spx_ogs_scatter_srb((int)type, k, &d, gv, v));
   } else {
     if (NrowBlocksT)
       SPX_CHECK(spx_ogs_scatter((int)type, k, NrowBlocksT, o_blockRowStartsT.ptr(), o_rowStartsT.ptr(),

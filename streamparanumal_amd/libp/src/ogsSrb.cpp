@@ -33,6 +33,7 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
   S.runBase.malloc(nch);
   S.linkMask.malloc(nch);
   S.linkBase.malloc(nch);
+  S.chunkRow.malloc(nch);
   memory<dlong> sorted(S.nnz > 0 ? S.nnz : 1);  // local index at each position
   memory<dlong> prevPos(S.nnz > 0 ? S.nnz : 1); // previous member's position, or -1
   std::vector<dlong> nruns(Nblocks, 0), nlinks(Nblocks, 0);
@@ -65,8 +66,9 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
           prevPos[e0 + pos] = (g == rowStarts[r]) ? -1 : posOfSlot[g - 1 - e0];
           links += (g != rowStarts[r]);
         }
-      dlong runs = 0;
+      dlong runs = 0, leading = 0;
       for (dlong c = 0; c * 64 < n; ++c) {
+        S.chunkRow[S.chunkStarts[b] + c] = blockStarts[b] + leading;
         uint64_t rmask = 0, lmask = 0;
         for (dlong l = 0; l < 64 && c * 64 + l < n; ++l) {
           const dlong i = e0 + c * 64 + l;
@@ -75,6 +77,7 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
             ++runs;
           }
           if (prevPos[i] >= 0) lmask |= 1ull << l;
+          else ++leading;
         }
         S.runMask[S.chunkStarts[b] + c] = rmask;
         S.linkMask[S.chunkStarts[b] + c] = lmask;
@@ -122,6 +125,7 @@ void SortedRunBlocks::Upload(platform_t &platform) {
   o_linkMask = platform.malloc<uint64_t>(linkMask);
   o_linkBase = platform.malloc<dlong>(linkBase);
   o_links = platform.malloc<uint16_t>(links);
+  o_chunkRow = platform.malloc<dlong>(chunkRow);
 }
 
 spx_srb_t SortedRunBlocks::Device() const {
@@ -138,6 +142,7 @@ spx_srb_t SortedRunBlocks::Device() const {
   d.linkMask = o_linkMask.ptr();
   d.linkBase = o_linkBase.ptr();
   d.links = o_links.ptr();
+  d.chunkRow = o_chunkRow.ptr();
   return d;
 }
 

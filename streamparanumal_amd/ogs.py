@@ -57,7 +57,8 @@ def build_csr(ids, kind=_lib.SIGNED, unique=True):
         libspxhost().spxh_csr_destroy(h)
 
 
-SRB_KEYS = ("entryStarts", "chunkStarts", "runMask", "runBase", "runIds", "linkMask", "linkBase", "links")
+SRB_KEYS = ("entryStarts", "chunkStarts", "runMask", "runBase", "runIds", "linkMask", "linkBase", "links",
+            "chunkRow")
 
 
 def srb_build(row_starts, col_ids, block_starts, cap):
@@ -75,7 +76,8 @@ def srb_build(row_starts, col_ids, block_starts, cap):
         out = {"entryStarts": np.zeros(nb + 1, np.int32), "chunkStarts": np.zeros(nb + 1, np.int32),
                "runMask": np.zeros(max(nch, 1), np.uint64), "runBase": np.zeros(max(nch, 1), np.int32),
                "runIds": np.zeros(max(nruns, 1), np.int32), "linkMask": np.zeros(max(nch, 1), np.uint64),
-               "linkBase": np.zeros(max(nch, 1), np.int32), "links": np.zeros(nlinks + 1, np.uint16)}
+               "linkBase": np.zeros(max(nch, 1), np.int32), "links": np.zeros(nlinks + 1, np.uint16),
+               "chunkRow": np.zeros(max(nch, 1), np.int32)}
         checkh(libspxhost().spxh_srb_copy(h, *(out[k].ctypes.data for k in SRB_KEYS)), "spxh_srb_copy")
         out.update(nblocks=nb, nnz=nnz, nchunks=nch, nruns=nruns, nlinks=nlinks, max_block_nnz=capv)
         return out
@@ -95,6 +97,14 @@ class SrbDevice:
     def gather_scatter(self, q, k=1, op=_lib.ADD):
         check(libspx().spx_ogs_gather_scatter_srb(ogs_type(q.dtype), op, k, C.byref(self.desc), q.ptr),
               "spx_ogs_gather_scatter_srb")
+
+    def gather(self, gq, q, k=1, op=_lib.ADD):
+        check(libspx().spx_ogs_gather_srb(ogs_type(q.dtype), op, k, C.byref(self.desc), q.ptr, gq.ptr),
+              "spx_ogs_gather_srb")
+
+    def scatter(self, q, gq, k=1):
+        check(libspx().spx_ogs_scatter_srb(ogs_type(q.dtype), k, C.byref(self.desc), gq.ptr, q.ptr),
+              "spx_ogs_scatter_srb")
 
 
 def block_rows(row_starts, cap):
@@ -162,6 +172,13 @@ class Ogs:
         xcd_remap: sorted-run launch hint (-1 library default)."""
         checkh(libspxhost().spxh_ogs_set_gs_format(self.h, 1 if sorted_runs else 0, xcd_remap),
                "spxh_ogs_set_gs_format")
+
+    def set_gather_format(self, gather_sorted=None, scatter_sorted=None):
+        """T-CSR gather (Trans) and scatter (NoTrans) on the sorted-run blocks (True) or the T-CSR row
+        blocks (False); None keeps the current choice (defaults: gather CSR, scatter sorted)."""
+        enc = lambda x: -1 if x is None else (1 if x else 0)
+        checkh(libspxhost().spxh_ogs_set_gather_format(self.h, enc(gather_sorted), enc(scatter_sorted)),
+               "spxh_ogs_set_gather_format")
 
     def autotune(self, dtype=np.float64, k=1, reps=5, mixed=False):
         """Time every launch shape of the Sym gather+scatter for k fields of dtype (mixed=True: the

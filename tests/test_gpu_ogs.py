@@ -179,6 +179,48 @@ def test_srb_equals_csr_format(setups, oracle, mesh, dtype, variant, cap, xcd):
         device.set_tuning("ogs_gs_srb", variant=-1)
 
 
+@pytest.mark.parametrize("mesh", MESHES)
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("lanes,cap", [(64, 512), (256, 2048), (64, 256)])
+def test_srb_gather_scatter_equal_csr(setups, oracle, mesh, dtype, lanes, cap):
+    """T-CSR gather (Trans, every op) and scatter (NoTrans) on the sorted-run layout: same bits as
+    the CSR kernels and the oracle, K = 1 and 3."""
+    say("srb gather/scatter", mesh, dtype.__name__, lanes, cap)
+    g, o, _ = setups[mesh]
+    g.set_block_nnz(2048, gs_cap=cap)
+    g.set_gs_format(True, -1)
+    device.set_tuning("ogs_gs_srb", variant=2 if lanes == 64 else 0)
+    try:
+        for k in (1, 3):
+            q = values(g.N, dtype, 29, k)
+            for op in OPS:
+                outs = []
+                for fmt in (True, False):
+                    g.set_gather_format(fmt, fmt)
+                    dq = DeviceArray.from_numpy(q)
+                    dg = DeviceArray(g.NlocalT * k, dtype)
+                    g.gather(dg, dq, k, op, spx.TRANS)
+                    device.check_kernel_errors()
+                    outs.append(dg.to_numpy())
+                assert same(outs[0], outs[1])
+                assert same(outs[0], o.gather(q, k, op, spx.TRANS))
+            gsrc = values(g.NlocalT, dtype, 31, k)
+            base = values(g.N, dtype, 32, k)
+            outs = []
+            for fmt in (True, False):
+                g.set_gather_format(fmt, fmt)
+                dv = DeviceArray.from_numpy(base)
+                g.scatter(dv, DeviceArray.from_numpy(gsrc), k, spx.NOTRANS)
+                device.check_kernel_errors()
+                outs.append(dv.to_numpy())
+            assert same(outs[0], outs[1])
+            assert same(outs[0], o.scatter(gsrc, base, k, spx.NOTRANS))
+    finally:
+        g.set_gather_format(False, True)
+        g.set_block_nnz(2048, gs_cap=512)
+        device.set_tuning("ogs_gs_srb", variant=-1)
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2])
 @pytest.mark.parametrize("cap", [64, 700, 4096])
 def test_srb_direct_random_ids(dev, oracle, seed, cap):
@@ -197,6 +239,15 @@ def test_srb_direct_random_ids(dev, oracle, seed, cap):
     d.gather_scatter(dq)
     device.check_kernel_errors()
     assert same(dq.to_numpy(), oracle.csr_gs(rs, ci, q))
+    # gather then scatter on the layout reproduces the gather+scatter
+    nrows = rs.size - 1
+    dg = DeviceArray(nrows)
+    d.gather(dg, DeviceArray.from_numpy(q))
+    dq2 = DeviceArray(n)
+    dq2.fill_zero()
+    d.scatter(dq2, dg)
+    device.check_kernel_errors()
+    assert same(dq2.to_numpy(), oracle.csr_gs(rs, ci, q))
 
 
 def test_reference_bisection_blocks(setups, oracle):

@@ -139,6 +139,17 @@ def test_srb_layout_matches_csr(mesh, cap):
             assert walk == [int(x) for x in row]
     for c in range(s["nchunks"]):
         assert int(s["runMask"][c]) & 1
+    # row of a row-leading position: chunkRow of its chunk plus the leading positions before it
+    for b in range(bs.size - 1):
+        e0, e1 = rs[bs[b]], rs[bs[b + 1]]
+        lead_rows = []
+        for pos in range(e1 - e0):
+            c = s["chunkStarts"][b] + (pos >> 6)
+            if prev[e0 + pos] < 0:
+                lm = int(s["linkMask"][c])
+                before = (pos & 63) - bin(lm & ((1 << (pos & 63)) - 1)).count("1")
+                lead_rows.append(int(s["chunkRow"][c]) + before)
+        assert lead_rows == list(range(bs[b], bs[b + 1]))
 
 
 def test_srb_rejects_bad_blocks():
