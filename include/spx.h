@@ -200,47 +200,46 @@ int spx_ogs_scatter_inverse(int type, int K, int32_t n, const int32_t *g2l, cons
 /* extract (ogsKernels.okl:167-177): gq[k + n*K] = q[k + ids[n]*K], n < n_ids */
 int spx_ogs_extract(int type, int K, int32_t n_ids, const int32_t *ids, const void *q, void *gq);
 
-/* Sorted-run row blocks: the T-CSR of a Sym operator re-listed for
+/* Sorted-run row blocks ("tiles"): the T-CSR of an operator re-listed for
    coalesced access (no reference counterpart; the rows, and the member order
    within each row, are those of the CSR of ogsOperator.hpp:44-70; built by
-   spxh_srb_build / the ogs_t setup).  Block b holds whole rows; its entries
-   [entryStarts[b], entryStarts[b+1]) are listed in ascending local index
-   ("positions" 0..n_b-1 of the block) and cut into chunks of 64: chunks
-   [chunkStarts[b], chunkStarts[b+1]), one per 64 positions.
-     runs:  bit l of runMask[c] is set when position 64c'+l (c' the chunk's
-            index within its block) starts a run of consecutive local indices
-            (bit 0 always); the chunk's runs begin at local indices
-            runIds[runBase[c]], runIds[runBase[c]+1], ...
-     rows:  bit l of linkMask[c] is set when that position is not the first
+   spxh_srb_build / the ogs_t setup).  Tile b holds whole rows; its entries are
+   listed in ascending local index ("positions") and it owns the chunk records
+   [b*chunks_per_tile, (b+1)*chunks_per_tile), record k describing positions
+   64k..64k+63, of which the first nvalid are entries.  In record C:
+     runs:  bit l of C.runMask is set when position l starts a run of
+            consecutive local indices (bit 0 always); the runs begin at local
+            indices runIds[C.runBase], runIds[C.runBase+1], ...
+     rows:  bit l of C.linkMask is set when position l is not the first
             (smallest) member of its row; such positions, in order, have the
-            block position of the previous member of their row in
-            links[linkBase[c]], links[linkBase[c]+1], ...; links ends with
-            one padding word (read, never used, for chunks without links)
-   Row values are folded from the first member along these links, i.e. in
-   ascending local index, as the CSR kernel does.  Arrays are device memory. */
+            tile position of the previous member of their row in
+            links[C.linkBase], links[C.linkBase+1], ...; links ends with one
+            padding word.
+     chunkRow: CSR row of the first row-leading position at or after
+            position 0 of the record (row-leading positions hold the rows in
+            CSR order).
+   Row values are folded from the first member along the links, i.e. in
+   ascending local index, as the CSR kernels do.  Arrays are device memory. */
+typedef struct {
+  uint64_t runMask, linkMask;
+  int32_t runBase, linkBase, chunkRow, nvalid;
+} spx_srb_chunk_t;
 typedef struct {
   int32_t nblocks;
-  int32_t max_block_nnz; /* <= 4096 */
-  int32_t xcd_remap;     /* 1: consecutive blocks on one XCD, 0: hardware order, -1: library default */
-  int32_t tile_lanes;    /* lanes per block: 64 (one-wave tiles, max_block_nnz <= 1024) or 256; 0: library default */
-  const int32_t *entryStarts;
-  const int32_t *chunkStarts;
-  const uint64_t *runMask;
-  const int32_t *runBase;
+  int32_t chunks_per_tile; /* 1..64: positions per tile = 64 * chunks_per_tile <= 4096 */
+  int32_t xcd_remap;       /* 1: consecutive tiles on one XCD, 0: hardware order, -1: library default */
+  int32_t tile_lanes;      /* lanes per tile: 64 (one wave, <= 1024 positions) or 256; 0: library default */
+  const spx_srb_chunk_t *chunks;
   const int32_t *runIds;
-  const uint64_t *linkMask;
-  const int32_t *linkBase;
   const uint16_t *links;
-  const int32_t *chunkRow; /* CSR row of the first row-leading position at or after each chunk's start
-                              (gather / scatter on the layout; may be NULL for gather+scatter) */
 } spx_srb_t;
 /* gatherScatter, ogs::Sym (ogsKernels.okl:32-81), on the sorted-run layout:
    same result as spx_ogs_gather_scatter on the CSR it was built from */
 int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, void *q);
 /* gather (ogsKernels.okl:86-122) and scatter (ogsKernels.okl:127-164) on the
    sorted-run layout of a CSR: same results as spx_ogs_gather /
-   spx_ogs_scatter on that CSR; the row of a position is chunkRow[c] plus the
-   row-leading positions before it in its chunk */
+   spx_ogs_scatter on that CSR; the row of a row-leading position is its
+   record's chunkRow plus the row-leading positions before it in the record */
 int spx_ogs_gather_srb(int type, int op, int K, const spx_srb_t *srb, const void *v, void *gv);
 int spx_ogs_scatter_srb(int type, int K, const spx_srb_t *srb, const void *gv, void *v);
 

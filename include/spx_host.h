@@ -69,7 +69,7 @@ int spxh_ogs_set_scatter_path(spxh_ogs_t ogs, int by_rows);
 int spxh_ogs_set_gs_format(spxh_ogs_t ogs, int sorted, int xcd_remap);
 /* layouts of the T-CSR gather (Trans) and scatter (NoTrans): 1 = the
    sorted-run blocks of the gather+scatter, 0 = T-CSR row blocks, -1 = keep;
-   defaults: gather on the CSR, scatter sorted.  Results are identical. */
+   both sorted by default.  Results are identical. */
 int spxh_ogs_set_gather_format(spxh_ogs_t ogs, int gather_sorted, int scatter_sorted);
 /* Autotuner (no reference counterpart; the reference's ogs::Auto times its
    halo exchange methods, ogsSetup.cpp:116-125, the same way): times every
@@ -119,20 +119,17 @@ int spxh_csr_destroy(spxh_csr_t csr);
 int spxh_block_rows(int32_t nrows, const int32_t *rowStarts, int32_t cap, int32_t *nblocks,
                     int32_t *blockStarts);
 
-/* Sorted-run row blocks (spx_srb_t) of a CSR and its row blocks, on the host
-   (no device needed): sizes[0..5] = nblocks, nnz, nchunks, nruns, nlinks,
-   max_block_nnz.  spxh_srb_copy fills host arrays of nblocks+1 (entryStarts,
-   chunkStarts), nchunks (runMask, runBase, linkMask, linkBase, chunkRow), nruns
-   (runIds) and nlinks + 1 (links, the last a padding word) entries (any may
-   be NULL); the caller uploads them and fills spx_srb_t with the device
-   addresses. */
+/* Sorted-run tiles (spx_srb_t) of a CSR and its row blocks, on the host (no
+   device needed): sizes[0..5] = nblocks, nnz, nchunks (= nblocks *
+   chunks_per_tile), nruns, nlinks, chunks_per_tile.  spxh_srb_copy fills
+   host arrays of nchunks records, nruns run ids, nlinks + 1 links (the last a
+   padding word) and nblocks + 1 CSR entry offsets of the tiles (any may be
+   NULL); the caller uploads chunks, runIds and links and fills spx_srb_t. */
 typedef struct spxh_srb_s *spxh_srb_t;
 int spxh_srb_build(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int32_t nblocks,
                    const int32_t *blockStarts, int32_t max_block_nnz, spxh_srb_t *out);
 int spxh_srb_sizes(spxh_srb_t srb, int64_t *sizes);
-int spxh_srb_copy(spxh_srb_t srb, int32_t *entryStarts, int32_t *chunkStarts, uint64_t *runMask,
-                  int32_t *runBase, int32_t *runIds, uint64_t *linkMask, int32_t *linkBase, uint16_t *links,
-                  int32_t *chunkRow);
+int spxh_srb_copy(spxh_srb_t srb, spx_srb_chunk_t *chunks, int32_t *runIds, uint16_t *links, int32_t *entryStarts);
 int spxh_srb_destroy(spxh_srb_t srb);
 
 /* mesh_t hex box ids: lexicographic GLL-grid ids of an nx x ny x nz box at

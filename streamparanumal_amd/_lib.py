@@ -67,10 +67,9 @@ class CgTuning(C.Structure):
 
 
 class Srb(C.Structure):
-    """spx_srb_t: device addresses of a sorted-run row block layout."""
-    _fields_ = [("nblocks", i32), ("max_block_nnz", i32), ("xcd_remap", i32), ("tile_lanes", i32), ("entryStarts", vp),
-                ("chunkStarts", vp), ("runMask", vp), ("runBase", vp), ("runIds", vp), ("linkMask", vp),
-                ("linkBase", vp), ("links", vp), ("chunkRow", vp)]
+    """spx_srb_t: device addresses of a sorted-run tile layout."""
+    _fields_ = [("nblocks", i32), ("chunks_per_tile", i32), ("xcd_remap", i32), ("tile_lanes", i32),
+                ("chunks", vp), ("runIds", vp), ("links", vp)]
 
 
 # name: (restype, argtypes) -- every function of include/spx.h
@@ -159,7 +158,7 @@ SPXH_FUNCS = {
                                          C.POINTER(CgTuning)]),
     "spxh_srb_build": (C.c_int, [i32, vp, vp, i32, vp, i32, C.POINTER(vp)]),
     "spxh_srb_sizes": (C.c_int, [vp, vp]),
-    "spxh_srb_copy": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "spxh_srb_copy": (C.c_int, [vp, vp, vp, vp, vp]),
     "spxh_srb_destroy": (C.c_int, [vp]),
     "spxh_ogs_gather": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_ogs_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp]),

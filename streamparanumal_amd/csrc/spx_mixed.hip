@@ -375,34 +375,27 @@ int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, v
   if (storage != SPX_STORE_F32 && storage != SPX_STORE_BF16)
     return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: unknown storage type");
   if (K < 1 || K > 65535) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: bad K");
-  if (srb->nblocks < 0 || srb->max_block_nnz < 1 || srb->max_block_nnz > 4096)
-    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: bad blocks");
+  if (srb->nblocks < 0 || srb->chunks_per_tile < 1 || srb->chunks_per_tile > 64)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: bad tiles");
   if (srb->nblocks == 0) return SPX_OK;
-  if (!srb->entryStarts || !srb->chunkStarts || !srb->runMask || !srb->runBase || !srb->runIds ||
-      !srb->linkMask || !srb->linkBase || !srb->links || !q)
+  if (!srb->chunks || !srb->runIds || !srb->links || !q)
     return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: null pointer");
-  const spx_tuning_t tu = tuning_for("ogs_gs_srb", spx_tuning_t{0, 1, 0, 0, 0, 2});
+  if (srb->tile_lanes != 0 && srb->tile_lanes != 64 && srb->tile_lanes != 256)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile_lanes must be 0, 64 or 256");
+  const spx_tuning_t tu = tuning_for("ogs_gs_srb", spx_tuning_t{0, 1, 0, 0, 0, 0});
   Launch L;
   L.nblocks = srb->nblocks;
   L.K = K;
   L.nxcd = (srb->xcd_remap >= 0 ? srb->xcd_remap : tu.xcd_remap) ? 8 : 1;
-  L.cap = srb->max_block_nnz;
+  L.cap = 64 * srb->chunks_per_tile;
   L.kf = (K >= 2 && K <= 4) ? K : 1;
   L.per = 0;
-  L.variant = tu.variant;
-  if (srb->tile_lanes != 0 && srb->tile_lanes != 64 && srb->tile_lanes != 256)
-    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile_lanes must be 0, 64 or 256");
-  if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
+  L.variant = srb->tile_lanes ? (srb->tile_lanes == 64 ? 2 : 0) : tu.variant;
   L.stream = current_stream();
   L.err = kernel_error_flag();
   SrbArgs A;
-  A.entryStarts = srb->entryStarts;
-  A.chunkStarts = srb->chunkStarts;
-  A.runMask = reinterpret_cast<const unsigned long long *>(srb->runMask);
-  A.runBase = srb->runBase;
+  A.chunks = reinterpret_cast<const SrbChunk *>(srb->chunks);
   A.runIds = srb->runIds;
-  A.linkMask = reinterpret_cast<const unsigned long long *>(srb->linkMask);
-  A.linkBase = srb->linkBase;
   A.links = srb->links;
   const int rc = storage == SPX_STORE_F32 ? launch_gs_srb_mixed(L, A, (float *)q) : launch_gs_srb_mixed(L, A, (bf16_t *)q);
   if (rc) return rc;

@@ -23,15 +23,15 @@ size_t type_size(int type) {
 
 // Measured on MI355X (BS8, 64^3 hex at p=7).  CSR blocks: 16-bit row starts
 // in LDS (variant 3) 4.9-5.0 TB/s against 4.4-5.0 for ranges in registers.
-// Sorted-run blocks: one-wave tiles (variant 2) with XCD remap, 5.1 TB/s at
-// p=7 and 5.8 at p=15, ahead of 256-lane tiles (variant 0) at every degree
-// measured; the remap gains 5-40% there.
+// Sorted-run tiles: 256 lanes (variant 0) with XCD remap, 5.8 TB/s at 2048
+// positions, against 5.1 for one-wave tiles of 512 (variant 2); one-wave tiles
+// stay ahead only below p=5.  The remap gains 2-40%.
 spx_tuning_t ogs_defaults(const char *kname) {
   spx_tuning_t d{};
   const std::string k(kname);
   d.block_threads = kThreads;
   d.xcd_remap = k == "ogs_gs_srb" ? 1 : 0;
-  d.variant = k == "ogs_gs" ? 3 : k == "ogs_gs_srb" ? 2 : 0;
+  d.variant = k == "ogs_gs" ? 3 : 0;
   return d;
 }
 
@@ -151,31 +151,37 @@ int spx_ogs_gather_scatter(int type, int op, int K, int32_t nblocks, const int32
   return SPX_OK;
 }
 
+namespace {
+// common checks and launch shape of the sorted-run entry points
+int srb_launch_setup(const char *name, int type, int K, const spx_srb_t *srb, Launch &L, SrbArgs &A) {
+  if (!srb) return fail(SPX_ERR_ARG, std::string(name) + ": null layout");
+  if (srb->chunks_per_tile < 1 || srb->chunks_per_tile > kMaxBlockNnz / 64)
+    return fail(SPX_ERR_ARG, std::string(name) + ": chunks_per_tile must be in [1, 64]");
+  const int cap = 64 * srb->chunks_per_tile;
+  int rc = check_common(name, type, K, srb->nblocks, cap);
+  if (rc) return rc;
+  if (srb->nblocks == 0) return SPX_OK;
+  if (!srb->chunks || !srb->runIds || !srb->links) return fail(SPX_ERR_ARG, std::string(name) + ": null layout array");
+  if (srb->tile_lanes != 0 && srb->tile_lanes != 64 && srb->tile_lanes != 256)
+    return fail(SPX_ERR_ARG, std::string(name) + ": tile_lanes must be 0, 64 or 256");
+  if ((rc = make_launch(name, "ogs_gs_srb", srb->nblocks, K, cap, type, 1, L))) return rc;
+  if (srb->xcd_remap >= 0) L.nxcd = srb->xcd_remap ? 8 : 1;
+  if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
+  A.chunks = reinterpret_cast<const SrbChunk *>(srb->chunks);
+  A.runIds = srb->runIds;
+  A.links = srb->links;
+  return SPX_OK;
+}
+}  // namespace
+
 int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, void *q) {
   SPX_NEED_DEVICE();
-  if (!srb) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: null layout");
-  int rc = check_common("spx_ogs_gather_scatter_srb", type, K, srb->nblocks, srb->max_block_nnz);
+  Launch L;
+  SrbArgs A;
+  int rc = srb_launch_setup("spx_ogs_gather_scatter_srb", type, K, srb, L, A);
   if (rc || (rc = check_op("spx_ogs_gather_scatter_srb", op))) return rc;
   if (srb->nblocks == 0) return SPX_OK;
-  if (!srb->entryStarts || !srb->chunkStarts || !srb->runMask || !srb->runBase || !srb->runIds ||
-      !srb->linkMask || !srb->linkBase || !srb->links || !q)
-    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: null pointer");
-  Launch L;
-  if ((rc = make_launch("spx_ogs_gather_scatter_srb", "ogs_gs_srb", srb->nblocks, K, srb->max_block_nnz, type, 1, L)))
-    return rc;
-  if (srb->xcd_remap >= 0) L.nxcd = srb->xcd_remap ? 8 : 1;
-  if (srb->tile_lanes != 0 && srb->tile_lanes != 64 && srb->tile_lanes != 256)
-    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: tile_lanes must be 0, 64 or 256");
-  if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
-  SrbArgs A;
-  A.entryStarts = srb->entryStarts;
-  A.chunkStarts = srb->chunkStarts;
-  A.runMask = reinterpret_cast<const unsigned long long *>(srb->runMask);
-  A.runBase = srb->runBase;
-  A.runIds = srb->runIds;
-  A.linkMask = reinterpret_cast<const unsigned long long *>(srb->linkMask);
-  A.linkBase = srb->linkBase;
-  A.links = srb->links;
+  if (!q) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: null pointer");
   rc = by_type(type, [&](auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
     return launch_gs_srb(L, op, A, (T *)q);
@@ -184,32 +190,6 @@ int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, vo
   SPX_LAUNCHED("spx_ogs_gather_scatter_srb");
   return SPX_OK;
 }
-
-namespace {
-int srb_launch_setup(const char *name, int type, int K, const spx_srb_t *srb, Launch &L, SrbArgs &A) {
-  if (!srb) return fail(SPX_ERR_ARG, std::string(name) + ": null layout");
-  int rc = check_common(name, type, K, srb->nblocks, srb->max_block_nnz);
-  if (rc) return rc;
-  if (srb->nblocks == 0) return SPX_OK;
-  if (!srb->entryStarts || !srb->chunkStarts || !srb->runMask || !srb->runBase || !srb->runIds || !srb->linkMask ||
-      !srb->linkBase || !srb->links || !srb->chunkRow)
-    return fail(SPX_ERR_ARG, std::string(name) + ": null layout array (chunkRow is needed)");
-  if ((rc = make_launch(name, "ogs_gs_srb", srb->nblocks, K, srb->max_block_nnz, type, 1, L))) return rc;
-  if (srb->xcd_remap >= 0) L.nxcd = srb->xcd_remap ? 8 : 1;
-  if (srb->tile_lanes != 0 && srb->tile_lanes != 64 && srb->tile_lanes != 256)
-    return fail(SPX_ERR_ARG, std::string(name) + ": tile_lanes must be 0, 64 or 256");
-  if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
-  A.entryStarts = srb->entryStarts;
-  A.chunkStarts = srb->chunkStarts;
-  A.runMask = reinterpret_cast<const unsigned long long *>(srb->runMask);
-  A.runBase = srb->runBase;
-  A.runIds = srb->runIds;
-  A.linkMask = reinterpret_cast<const unsigned long long *>(srb->linkMask);
-  A.linkBase = srb->linkBase;
-  A.links = srb->links;
-  return SPX_OK;
-}
-}  // namespace
 
 int spx_ogs_gather_srb(int type, int op, int K, const spx_srb_t *srb, const void *v, void *gv) {
   SPX_NEED_DEVICE();
@@ -221,7 +201,7 @@ int spx_ogs_gather_srb(int type, int op, int K, const spx_srb_t *srb, const void
   if (!v || !gv) return fail(SPX_ERR_ARG, "spx_ogs_gather_srb: null pointer");
   rc = by_type(type, [&](auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
-    return launch_gather_srb(L, op, A, srb->chunkRow, (const T *)v, (T *)gv);
+    return launch_gather_srb(L, op, A, (const T *)v, (T *)gv);
   });
   if (rc) return rc;
   SPX_LAUNCHED("spx_ogs_gather_srb");
@@ -238,7 +218,7 @@ int spx_ogs_scatter_srb(int type, int K, const spx_srb_t *srb, const void *gv, v
   if (!v || !gv) return fail(SPX_ERR_ARG, "spx_ogs_scatter_srb: null pointer");
   rc = by_type(type, [&](auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
-    return launch_scatter_srb(L, A, srb->chunkRow, (const T *)gv, (T *)v);
+    return launch_scatter_srb(L, A, (const T *)gv, (T *)v);
   });
   if (rc) return rc;
   SPX_LAUNCHED("spx_ogs_scatter_srb");

@@ -57,33 +57,9 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
   });
 }
 
-int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q) {
-  const size_t lds = lds_bytes(L.cap, L.kf, sizeof(T), 1) + (size_t)L.cap * sizeof(unsigned short);
-  if (lds > 65536) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: tile too large for LDS");
-  if (L.variant == 2 && L.cap <= 16 * 64) {
-    // one-wave tiles: 64 lanes, cap <= 64*PER; fields one per pass when a
-    // multi-field tile does not fit one wave's registers (larger tiles take
-    // the 256-lane kernel below)
-    Launch W = L;
-    W.per = pick_per(W.kf, W.cap * (kThreads / 64));
-    if (!W.per) {
-      W.kf = 1;
-      W.per = pick_per(1, W.cap * (kThreads / 64));
-    }
-    if (!W.per) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: tile too large for one wave");
-    const size_t ldsw = lds_bytes(W.cap, W.kf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short);
-    return dispatch<true>(op, W.kf, W.per, [&]<int OPC, int KF, int PER>() {
-      hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), ldsw, W.stream, W, SRB_ARGS(A), q);
-      return SPX_OK;
-    });
-  }
-  return dispatch<true>(op, L.kf, L.per, [&]<int OPC, int KF, int PER>() {
-    hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER>), grid_of(L), dim3(kThreads), lds, L.stream, L, SRB_ARGS(A), q);
-    return SPX_OK;
-  });
-}
-
-// sorted-run gather / scatter: one-wave tiles (variant 2, cap <= 1024) or 256 lanes
+// sorted-run kernels: one-wave tiles (variant 2, up to 1024 positions) or 256
+// lanes; fields one per pass when a multi-field tile does not fit the lanes'
+// registers.  L.cap is the layout's positions per tile (chunks per tile * 64).
 template <bool WITH_OP, typename F>
 static int srb_shape(const Launch &L, int op, F &&f) {
   Launch W = L;
@@ -102,26 +78,35 @@ static int srb_shape(const Launch &L, int op, F &&f) {
   });
 }
 
-int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const int *chunkRow, const T *v, T *gv) {
+int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q) {
   return srb_shape<true>(L, op, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
     if (wave)
-      hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A),
-                         (T *)nullptr, chunkRow, v, gv);
+      hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A), q);
     else
-      hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A),
-                         (T *)nullptr, chunkRow, v, gv);
+      hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), q);
     return SPX_OK;
   });
 }
 
-int launch_scatter_srb(const Launch &L, const SrbArgs &A, const int *chunkRow, const T *gv, T *v) {
+int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const T *v, T *gv) {
+  return srb_shape<true>(L, op, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
+    if (wave)
+      hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A), v,
+                         gv);
+    else
+      hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), v,
+                         gv);
+    return SPX_OK;
+  });
+}
+
+int launch_scatter_srb(const Launch &L, const SrbArgs &A, const T *gv, T *v) {
   return srb_shape<false>(L, SPX_ADD, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
     if (wave)
-      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A),
-                         (T *)nullptr, chunkRow, gv, v);
+      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A), gv, v);
     else
-      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A),
-                         (T *)nullptr, chunkRow, gv, v);
+      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), gv,
+                         v);
     return SPX_OK;
   });
 }

@@ -214,83 +214,110 @@ __global__ __launch_bounds__(kThreads) void k_gs_sym(Launch L, const int *__rest
       }
 }
 
-// ----------------------------------------------------- Sym GS, sorted runs --
-// The block's entries in ascending local index ("positions"): lane loads and
-// stores hit runs of consecutive nodes.  Position pos, chunk c = pos/64,
-// l = pos%64, decodes as
-//   local index = runIds[runBase[c] + popc(runMask[c] & bits<=l) - 1] + (l - top)
-//     where top is the highest set bit of runMask[c] at or below l;
-//   previous member of its row (when bit l of linkMask[c] is set)
-//              = links[linkBase[c] + popc(linkMask[c] & bits<l)].
-// Values sit in LDS at their positions; next[] links each member to the
-// following one, and the first member of each row folds the row along them
-// (ascending local index, the CSR order) and writes the result back.  NT = 64
-// gives one-wave tiles: more independent tiles per CU, no cross-wave barrier.
+// ----------------------------------------------------- sorted-run tiles --
+// Tile b (a row block) lists its entries in ascending local index
+// ("positions") and owns the chunk records [b*cpt, (b+1)*cpt), record k for
+// positions 64k..64k+63, the first nvalid of which are entries.  Position
+// l of record C decodes as
+//   local index = runIds[C.runBase + popc(C.runMask & bits<=l) - 1] + (l - top)
+//     where top is the highest set bit of C.runMask at or below l;
+//   previous member of its row (bit l of C.linkMask set)
+//              = links[C.linkBase + popc(C.linkMask & bits<l)];
+//   row of a row-leading position = C.chunkRow + (l - popc(C.linkMask & bits<l)).
+// Records are found from the tile index alone, so the first loads of a tile
+// need no header.  Values sit in LDS at their positions; next[] links each
+// member to the following one, and the first member of each row folds the
+// row along them (ascending local index, the CSR order).  NT = 64 gives
+// one-wave tiles: more independent tiles per CU, no cross-wave barrier.
+struct SrbChunk {
+  unsigned long long runMask, linkMask;
+  int runBase, linkBase, chunkRow, nvalid;
+};
+static_assert(sizeof(SrbChunk) == 32, "chunk record layout");
+
 struct SrbArgs {
-  const int *entryStarts, *chunkStarts;
-  const unsigned long long *runMask, *linkMask;
-  const int *runBase, *runIds, *linkBase;
+  const SrbChunk *chunks;
+  const int *runIds;
   const unsigned short *links;
 };
 
 constexpr unsigned short kEnd = 0xffff;
 
-// The layout arrays as separate restrict kernel arguments: the compiler may
-// then read the block's uniform words through the scalar cache even though
-// the kernel stores to q.
-#define SRB_PARAMS                                                                                            \
-  const int *__restrict__ entryStarts, const int *__restrict__ chunkStarts,                                    \
-      const unsigned long long *__restrict__ runMask, const int *__restrict__ runBase,                         \
-      const int *__restrict__ runIds, const unsigned long long *__restrict__ linkMask,                         \
-      const int *__restrict__ linkBase, const unsigned short *__restrict__ links, T *__restrict__ q
-#define SRB_ARGS(A) A.entryStarts, A.chunkStarts, A.runMask, A.runBase, A.runIds, A.linkMask, A.linkBase, A.links
+// The layout arrays as restrict kernel arguments: the compiler may then read
+// the wave-uniform chunk records through the scalar cache although the
+// kernel stores to its value arrays.
+#define SRB_PARAMS \
+  const SrbChunk *__restrict__ chunks, const int *__restrict__ runIds, const unsigned short *__restrict__ links
+#define SRB_ARGS(A) A.chunks, A.runIds, A.links
+
+// Positions j*NT + threadIdx.x of tile b: local index, previous member (-1:
+// row-leading or not an entry), CSR row (row-leading positions), and validity
+// bits.  Chunk records past the tile's cpt are replaced by its last one and
+// their positions marked invalid, so every load stays inside the layout.
+template <int PER, int NT>
+__device__ __forceinline__ void srb_decode(const SrbChunk *__restrict__ chunks, const int *__restrict__ runIds,
+                                           const unsigned short *__restrict__ links, int b, int cpt,
+                                           int (&id)[PER], int (&prev)[PER], int (&row)[PER], unsigned &valid) {
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int l = t & 63;
+  // The wave's PER records are 8*PER dwords: lane l fetches dword l%8 of
+  // record l/8 (+8 per extra round) in one coalesced load; the fields then
+  // reach every lane by readlane.  (Scalar loads of the records would return
+  // out of order, and each use would wait for all of them.)
+  constexpr int ROUNDS = (PER * 8 + 63) / 64;
+  const int *words = reinterpret_cast<const int *>(chunks);
+  int word[ROUNDS];
+#pragma unroll
+  for (int h = 0; h < ROUNDS; ++h) {
+    const int jr = min(h * 8 + (l >> 3), PER - 1);
+    const int kc = min((NT / 64) * jr + w, cpt - 1);
+    word[h] = words[((int64_t)b * cpt + kc) * 8 + (l & 7)];
+  }
+  valid = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int k = (NT / 64) * j + w;  // wave-uniform
+    const int src = (j & 7) * 8;
+    const int wv = word[j / 8];
+    const unsigned long long rm = (unsigned)__builtin_amdgcn_readlane(wv, src + 0) |
+                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(wv, src + 1) << 32);
+    const unsigned long long lm = (unsigned)__builtin_amdgcn_readlane(wv, src + 2) |
+                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(wv, src + 3) << 32);
+    const int runBase = __builtin_amdgcn_readlane(wv, src + 4);
+    const int linkBase = __builtin_amdgcn_readlane(wv, src + 5);
+    const int chunkRow = __builtin_amdgcn_readlane(wv, src + 6);
+    const int nv = k < cpt ? min(__builtin_amdgcn_readlane(wv, src + 7), 64) : 0;
+    const bool ok = l < nv;
+    const int ls = ok ? l : 0;  // bit 0 of runMask is always set
+    const unsigned long long upto = (2ull << ls) - 1ull;
+    const unsigned long long rbel = rm & upto;
+    id[j] = runIds[runBase + __builtin_popcountll(rbel) - 1] + (ls - (63 - __builtin_clzll(rbel)));
+    const int before = __builtin_popcountll(lm & (upto >> 1));
+    const bool linked = ok && ((lm >> ls) & 1ull);
+    prev[j] = linked ? (int)links[linkBase + before] : -1;
+    row[j] = chunkRow + (ls - before);
+    valid |= (ok ? 1u : 0u) << j;
+  }
+}
 
 template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>>
-__global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS) {
-  const SrbArgs A{entryStarts, chunkStarts, runMask, linkMask, runBase, runIds, linkBase, links};
+__global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restrict__ q) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
   unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
   const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
   const int kb = blockIdx.y * KF;
-  const int e0 = A.entryStarts[b];
-  const int nnz = A.entryStarts[b + 1] - e0;
-  const int c0 = A.chunkStarts[b];
-  const int nch = A.chunkStarts[b + 1] - c0;
-  if (nnz > L.cap || nch != (nnz + 63) / 64) {
-    flag_error(L.err, 6);
-    return;
-  }
-  if (nnz == 0) return;
+  const int cap = L.cap;  // cpt * 64 positions
   const int64_t K = L.K;
   const int t = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz) next[t + j * NT] = kEnd;
-  unsigned long long rm[PER], lm[PER];
-  int rb[PER], lb[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int c = c0 + min((NT / 64) * j + w, nch - 1);  // wave-uniform
-    rm[j] = A.runMask[c];
-    lm[j] = A.linkMask[c];
-    rb[j] = A.runBase[c];
-    lb[j] = A.linkBase[c];
-  }
-  int id[PER], prev[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int l = clampi(t + j * NT, nnz - 1) & 63;
-    const unsigned long long upto = (2ull << l) - 1ull;
-    const unsigned long long rbel = rm[j] & upto;
-    const int k = __builtin_popcountll(rbel) - 1;
-    const int top = 63 - __builtin_clzll(rbel);
-    id[j] = A.runIds[rb[j] + k] + (l - top);
-    const bool linked = (lm[j] >> l) & 1ull;
-    const int li = lb[j] + __builtin_popcountll(lm[j] & (upto >> 1));
-    prev[j] = linked ? (int)A.links[li] : -1;
-  }
+    if (t + j * NT < cap) next[t + j * NT] = kEnd;
+  int id[PER], prev[PER], row[PER];
+  unsigned valid;
+  srb_decode<PER, NT>(chunks, runIds, links, b, cap / 64, id, prev, row, valid);
+  (void)row;
   T v[PER][KF];
 #pragma unroll
   for (int j = 0; j < PER; ++j)
@@ -299,83 +326,52 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS) {
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz) {
+    if ((valid >> j) & 1u) {
 #pragma unroll
       for (int f = 0; f < KF; ++f) vals[(t + j * NT) * KF + f] = v[j][f];
-      if (prev[j] >= 0 && prev[j] < nnz) next[prev[j]] = (unsigned short)(t + j * NT);
+      if (prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
     }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz && prev[j] < 0) {
+    if (((valid >> j) & 1u) && prev[j] < 0) {
       const int h = t + j * NT;
 #pragma unroll
       for (int f = 0; f < KF; ++f) {
         typename F::Acc acc = F::init();
         // walks end at kEnd (> cap); the step bound also ends them on a malformed layout
-        for (int i = h, s = 0; i < nnz && s < nnz; i = next[i], ++s) F::apply(acc, vals[i * KF + f]);
+        for (int i = h, s = 0; i < cap && s < cap; i = next[i], ++s) F::apply(acc, vals[i * KF + f]);
         const T r = F::out(acc);
-        for (int i = h, s = 0; i < nnz && s < nnz; i = next[i], ++s) vals[i * KF + f] = r;
+        for (int i = h, s = 0; i < cap && s < cap; i = next[i], ++s) vals[i * KF + f] = r;
       }
     }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz)
+    if ((valid >> j) & 1u)
 #pragma unroll
       for (int f = 0; f < KF; ++f) q[(int64_t)id[j] * K + kb + f] = vals[(t + j * NT) * KF + f];
 }
 
-// Gather and scatter on the sorted-run layout.  A position's row is
-// chunkRow[c] plus the row-leading (unlinked) positions before it in its
-// chunk; row-leading positions hold the rows in CSR order.
-template <int PER, int NT>
-__device__ __forceinline__ void srb_decode(const SrbArgs &A, const int *__restrict__ chunkRow, int c0, int nch,
-                                           int nnz, int (&id)[PER], int (&prev)[PER], int (&row)[PER]) {
-  const int t = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int c = c0 + min((NT / 64) * j + w, nch - 1);  // wave-uniform
-    const unsigned long long rm = A.runMask[c], lm = A.linkMask[c];
-    const int rbase = A.runBase[c], lbase = A.linkBase[c], crow = chunkRow[c];
-    const int l = clampi(t + j * NT, nnz - 1) & 63;
-    const unsigned long long upto = (2ull << l) - 1ull;
-    const unsigned long long rbel = rm & upto;
-    id[j] = A.runIds[rbase + __builtin_popcountll(rbel) - 1] + (l - (63 - __builtin_clzll(rbel)));
-    const int linkedBefore = __builtin_popcountll(lm & (upto >> 1));
-    const bool linked = (lm >> l) & 1ull;
-    prev[j] = linked ? (int)A.links[lbase + linkedBefore] : -1;
-    row[j] = crow + (l - linkedBefore);  // valid for row-leading positions
-  }
-}
-
+// gather (Trans) on the sorted-run layout: row-leading positions fold their
+// row along the links and write gv at the row's CSR index
 template <typename T, int OP, int KF, int PER, int NT = kThreads>
-__global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const int *__restrict__ chunkRow,
-                                                   const T *__restrict__ v, T *__restrict__ gv) {
-  const SrbArgs A{entryStarts, chunkStarts, runMask, linkMask, runBase, runIds, linkBase, links};
-  (void)q;
+__global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const T *__restrict__ v,
+                                                   T *__restrict__ gv) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
   unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
   const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
   const int kb = blockIdx.y * KF;
-  const int e0 = A.entryStarts[b];
-  const int nnz = A.entryStarts[b + 1] - e0;
-  const int c0 = A.chunkStarts[b];
-  const int nch = A.chunkStarts[b + 1] - c0;
-  if (nnz > L.cap || nch != (nnz + 63) / 64) {
-    flag_error(L.err, 7);
-    return;
-  }
-  if (nnz == 0) return;
+  const int cap = L.cap;
   const int64_t K = L.K;
   const int t = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz) next[t + j * NT] = kEnd;
+    if (t + j * NT < cap) next[t + j * NT] = kEnd;
   int id[PER], prev[PER], row[PER];
-  srb_decode<PER, NT>(A, chunkRow, c0, nch, nnz, id, prev, row);
+  unsigned valid;
+  srb_decode<PER, NT>(chunks, runIds, links, b, cap / 64, id, prev, row, valid);
   T x[PER][KF];
 #pragma unroll
   for (int j = 0; j < PER; ++j)
@@ -384,74 +380,66 @@ __global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const i
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz) {
+    if ((valid >> j) & 1u) {
 #pragma unroll
       for (int f = 0; f < KF; ++f) vals[(t + j * NT) * KF + f] = x[j][f];
-      if (prev[j] >= 0 && prev[j] < nnz) next[prev[j]] = (unsigned short)(t + j * NT);
+      if (prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
     }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz && prev[j] < 0) {
+    if (((valid >> j) & 1u) && prev[j] < 0) {
 #pragma unroll
       for (int f = 0; f < KF; ++f) {
         T acc = Fold<T, OP>::init();
-        for (int i = t + j * NT, s = 0; i < nnz && s < nnz; i = next[i], ++s) Fold<T, OP>::apply(acc, vals[i * KF + f]);
+        for (int i = t + j * NT, s = 0; i < cap && s < cap; i = next[i], ++s) Fold<T, OP>::apply(acc, vals[i * KF + f]);
         gv[(int64_t)row[j] * K + kb + f] = acc;
       }
     }
 }
 
+// scatter (NoTrans) on the sorted-run layout: row-leading positions read
+// their row's value and pass it along the links; every position stores
 template <typename T, int KF, int PER, int NT = kThreads>
-__global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const int *__restrict__ chunkRow,
-                                                    const T *__restrict__ gv, T *__restrict__ v) {
-  const SrbArgs A{entryStarts, chunkStarts, runMask, linkMask, runBase, runIds, linkBase, links};
-  (void)q;
+__global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const T *__restrict__ gv,
+                                                    T *__restrict__ v) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
   unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
   const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
   const int kb = blockIdx.y * KF;
-  const int e0 = A.entryStarts[b];
-  const int nnz = A.entryStarts[b + 1] - e0;
-  const int c0 = A.chunkStarts[b];
-  const int nch = A.chunkStarts[b + 1] - c0;
-  if (nnz > L.cap || nch != (nnz + 63) / 64) {
-    flag_error(L.err, 8);
-    return;
-  }
-  if (nnz == 0) return;
+  const int cap = L.cap;
   const int64_t K = L.K;
   const int t = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz) next[t + j * NT] = kEnd;
+    if (t + j * NT < cap) next[t + j * NT] = kEnd;
   int id[PER], prev[PER], row[PER];
-  srb_decode<PER, NT>(A, chunkRow, c0, nch, nnz, id, prev, row);
-  // every lane reads the value of its position's row when it leads the row
-  // (a clamped row otherwise: an in-bounds load whose value is dropped)
+  unsigned valid;
+  srb_decode<PER, NT>(chunks, runIds, links, b, cap / 64, id, prev, row, valid);
+  // only row-leading positions read (the row index of the others may lie
+  // one past the last row)
   T g[PER][KF];
 #pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int64_t r = prev[j] < 0 ? row[j] : (int64_t)chunkRow[c0];
+  for (int j = 0; j < PER; ++j)
 #pragma unroll
-    for (int f = 0; f < KF; ++f) g[j][f] = gv[r * K + kb + f];
-  }
+    for (int f = 0; f < KF; ++f)
+      g[j][f] = (((valid >> j) & 1u) && prev[j] < 0) ? gv[(int64_t)row[j] * K + kb + f] : T(0);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz && prev[j] >= 0 && prev[j] < nnz) next[prev[j]] = (unsigned short)(t + j * NT);
+    if (((valid >> j) & 1u) && prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz && prev[j] < 0)
+    if (((valid >> j) & 1u) && prev[j] < 0)
 #pragma unroll
       for (int f = 0; f < KF; ++f)
-        for (int i = t + j * NT, s = 0; i < nnz && s < nnz; i = next[i], ++s) vals[i * KF + f] = g[j][f];
+        for (int i = t + j * NT, s = 0; i < cap && s < cap; i = next[i], ++s) vals[i * KF + f] = g[j][f];
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (t + j * NT < nnz)
+    if ((valid >> j) & 1u)
 #pragma unroll
       for (int f = 0; f < KF; ++f) v[(int64_t)id[j] * K + kb + f] = vals[(t + j * NT) * KF + f];
 }
@@ -711,8 +699,8 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
   int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq);           \
   int launch_scatter_inv(hipStream_t s, int64_t n, int K, const int *g2l, const T *gv, T *v, int npl); \
   int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q);                                       \
-  int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const int *chunkRow, const T *v, T *gv); \
-  int launch_scatter_srb(const Launch &L, const SrbArgs &A, const int *chunkRow, const T *gv, T *v);
+  int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const T *v, T *gv);                      \
+  int launch_scatter_srb(const Launch &L, const SrbArgs &A, const T *gv, T *v);
 SPX_OGS_GEN_DECL(float)
 SPX_OGS_GEN_DECL(double)
 SPX_OGS_GEN_DECL(int)

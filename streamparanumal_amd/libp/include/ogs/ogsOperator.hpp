@@ -10,31 +10,41 @@ namespace libp {
 namespace ogs {
 
 // Reference defaults (ogsOperator.hpp:37-38) and this build's default tiles,
-// measured on MI355X at 64^3 hex p=7: gather/scatter fastest with 2048-entry
-// row blocks, the sorted-run gather+scatter with 512-entry (one-wave) tiles.
+// measured on MI355X at 64^3 hex p=7: CSR gather/scatter blocks of 2048
+// entries, sorted-run tiles of 2048 positions.
 constexpr int blockSize = 256;
 constexpr int gatherNodesPerBlock = 512;
 constexpr int defaultBlockNnz = 2048;
-constexpr int defaultGsBlockNnz = 512;
+constexpr int defaultGsBlockNnz = 2048;
 constexpr int maxBlockNnz = 4096;
 
-// Sorted-run row blocks of the T-CSR (spx_srb_t): each row block's entries
-// in ascending local index, coded per 64-entry chunk as runs of consecutive
-// indices, with a link from each row member to the previous one.
+// Sorted-run row blocks ("tiles") of the T-CSR (spx_srb_t): each tile's
+// entries in ascending local index, described by cpt chunk records of 64
+// positions (runs of consecutive indices, links from each row member to the
+// previous one, the CSR row of the first row-leading position).
+typedef spx_srb_chunk_t SrbChunkRecord;
+
 struct SortedRunBlocks {
-  dlong nblocks = 0, cap = 0, nnz = 0, nchunks = 0, nruns = 0, nlinks = 0;
-  int xcdRemap = -1;   // launch hints passed in spx_srb_t (-1 / 0: library default)
+  dlong nblocks = 0, cpt = 0, nnz = 0, nchunks = 0, nruns = 0, nlinks = 0;
+  int xcdRemap = -1;  // launch hints passed in spx_srb_t (-1 / 0: library default)
   int tileLanes = 0;
-  memory<dlong> entryStarts, chunkStarts, runBase, runIds, linkBase, chunkRow;
-  memory<uint64_t> runMask, linkMask;
+  memory<dlong> entryStarts;  // host only: CSR entries of each tile
+  memory<SrbChunkRecord> chunks;
+  memory<dlong> runIds;
   memory<uint16_t> links;
-  deviceMemory<dlong> o_entryStarts, o_chunkStarts, o_runBase, o_runIds, o_linkBase, o_chunkRow;
-  deviceMemory<uint64_t> o_runMask, o_linkMask;
+  deviceMemory<SrbChunkRecord> o_chunks;
+  deviceMemory<dlong> o_runIds;
   deviceMemory<uint16_t> o_links;
 
   void Upload(platform_t &platform);
   spx_srb_t Device() const;
 };
+
+// chunk records per tile for a row block capacity (a power of two >= 4)
+int SrbChunksPerTile(int cap);
+
+SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowStarts, const memory<dlong> colIds,
+                                     const dlong Nblocks, const memory<dlong> blockStarts, const int cap);
 
 // One launch shape of the Sym gather+scatter and its measured time.
 struct GsTuning {
@@ -59,8 +69,7 @@ struct CgTuning {
 CgTuning AutotuneCgBatch(const std::vector<int64_t> &segStarts, int storage, void *p, void *Ap, void *x, void *r,
                          void *rdotr, int reps, spx_cg_plan_t *plan, std::vector<CgTuning> *log = nullptr);
 
-SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowStarts, const memory<dlong> colIds,
-                                     const dlong Nblocks, const memory<dlong> blockStarts, const int cap);
+
 
 class ogsOperator_t {
  public:
@@ -91,9 +100,9 @@ class ogsOperator_t {
   bool gsSorted = true;
   // The T-CSR gather (Trans) and scatter (NoTrans) on the same layout or on
   // the T-CSR row blocks of blockNnz entries.  Measured on MI355X at 64^3 hex
-  // p=7: scatter 5.25 TB/s sorted against 4.32 on the CSR, gather 5.19
-  // against 5.54.
-  bool gatherSorted = false;
+  // p=7 (algorithmic TB/s): scatter 6.6 sorted against 4.3 on the CSR,
+  // gather 6.8 against 5.6.
+  bool gatherSorted = true;
   bool scatterSorted = true;
 
   // Local -> gathered row maps of the T- and N-CSR (-1: in no row), for the

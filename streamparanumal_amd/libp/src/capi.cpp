@@ -371,26 +371,20 @@ int spxh_srb_sizes(spxh_srb_t h, int64_t *sizes) {
   sizes[2] = s.nchunks;
   sizes[3] = s.nruns;
   sizes[4] = s.nlinks;
-  sizes[5] = s.cap;
+  sizes[5] = s.cpt;
   return SPX_OK;
 }
 
-int spxh_srb_copy(spxh_srb_t h, int32_t *entryStarts, int32_t *chunkStarts, uint64_t *runMask, int32_t *runBase,
-                  int32_t *runIds, uint64_t *linkMask, int32_t *linkBase, uint16_t *links, int32_t *chunkRow) {
+int spxh_srb_copy(spxh_srb_t h, spx_srb_chunk_t *chunks, int32_t *runIds, uint16_t *links, int32_t *entryStarts) {
   if (!h) return err(SPX_ERR_ARG, "spxh_srb_copy: null");
   const ogs::SortedRunBlocks &s = h->srb;
   auto cp = [](const auto &m, size_t n, auto *dst) {
     if (dst && n) std::copy(m.ptr(), m.ptr() + n, dst);
   };
-  cp(s.entryStarts, (size_t)s.nblocks + 1, entryStarts);
-  cp(s.chunkStarts, (size_t)s.nblocks + 1, chunkStarts);
-  cp(s.runMask, (size_t)s.nchunks, runMask);
-  cp(s.runBase, (size_t)s.nchunks, runBase);
+  cp(s.chunks, (size_t)s.nchunks, chunks);
   cp(s.runIds, (size_t)s.nruns, runIds);
-  cp(s.linkMask, (size_t)s.nchunks, linkMask);
-  cp(s.linkBase, (size_t)s.nchunks, linkBase);
   cp(s.links, (size_t)s.nlinks + 1, links);
-  cp(s.chunkRow, (size_t)s.nchunks, chunkRow);
+  cp(s.entryStarts, (size_t)s.nblocks + 1, entryStarts);
   return SPX_OK;
 }
 

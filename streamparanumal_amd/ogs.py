@@ -57,12 +57,13 @@ def build_csr(ids, kind=_lib.SIGNED, unique=True):
         libspxhost().spxh_csr_destroy(h)
 
 
-SRB_KEYS = ("entryStarts", "chunkStarts", "runMask", "runBase", "runIds", "linkMask", "linkBase", "links",
-            "chunkRow")
+# spx_srb_chunk_t as a numpy record
+SRB_CHUNK = np.dtype([("runMask", "<u8"), ("linkMask", "<u8"), ("runBase", "<i4"), ("linkBase", "<i4"),
+                      ("chunkRow", "<i4"), ("nvalid", "<i4")])
 
 
 def srb_build(row_starts, col_ids, block_starts, cap):
-    """Sorted-run row blocks (spx_srb_t) of a CSR and its row blocks, as host arrays."""
+    """Sorted-run tiles (spx_srb_t) of a CSR and its row blocks, as host arrays."""
     rs = np.ascontiguousarray(row_starts, dtype=np.int32)
     ci = np.ascontiguousarray(col_ids, dtype=np.int32)
     bs = np.ascontiguousarray(block_starts, dtype=np.int32)
@@ -72,14 +73,12 @@ def srb_build(row_starts, col_ids, block_starts, cap):
     try:
         sz = np.zeros(6, dtype=np.int64)
         checkh(libspxhost().spxh_srb_sizes(h, sz.ctypes.data), "spxh_srb_sizes")
-        nb, nnz, nch, nruns, nlinks, capv = (int(x) for x in sz)
-        out = {"entryStarts": np.zeros(nb + 1, np.int32), "chunkStarts": np.zeros(nb + 1, np.int32),
-               "runMask": np.zeros(max(nch, 1), np.uint64), "runBase": np.zeros(max(nch, 1), np.int32),
-               "runIds": np.zeros(max(nruns, 1), np.int32), "linkMask": np.zeros(max(nch, 1), np.uint64),
-               "linkBase": np.zeros(max(nch, 1), np.int32), "links": np.zeros(nlinks + 1, np.uint16),
-               "chunkRow": np.zeros(max(nch, 1), np.int32)}
-        checkh(libspxhost().spxh_srb_copy(h, *(out[k].ctypes.data for k in SRB_KEYS)), "spxh_srb_copy")
-        out.update(nblocks=nb, nnz=nnz, nchunks=nch, nruns=nruns, nlinks=nlinks, max_block_nnz=capv)
+        nb, nnz, nch, nruns, nlinks, cpt = (int(x) for x in sz)
+        out = {"chunks": np.zeros(max(nch, 1), SRB_CHUNK), "runIds": np.zeros(max(nruns, 1), np.int32),
+               "links": np.zeros(nlinks + 1, np.uint16), "entryStarts": np.zeros(nb + 1, np.int32)}
+        checkh(libspxhost().spxh_srb_copy(h, *(out[k].ctypes.data for k in ("chunks", "runIds", "links",
+                                                                                "entryStarts"))), "spxh_srb_copy")
+        out.update(nblocks=nb, nnz=nnz, nchunks=nch, nruns=nruns, nlinks=nlinks, chunks_per_tile=cpt)
         return out
     finally:
         libspxhost().spxh_srb_destroy(h)
@@ -90,9 +89,11 @@ class SrbDevice:
 
     def __init__(self, srb, xcd_remap=-1, tile_lanes=0):
         from .device import DeviceArray
-        self.arrays = {k: DeviceArray.from_numpy(srb[k]) for k in SRB_KEYS}
-        self.desc = _lib.Srb(srb["nblocks"], srb["max_block_nnz"], xcd_remap, tile_lanes,
-                             *(self.arrays[k].ptr for k in SRB_KEYS))
+        self.arrays = {"chunks": DeviceArray.from_numpy(srb["chunks"].view(np.uint8)),
+                       "runIds": DeviceArray.from_numpy(srb["runIds"]),
+                       "links": DeviceArray.from_numpy(srb["links"])}
+        self.desc = _lib.Srb(srb["nblocks"], srb["chunks_per_tile"], xcd_remap, tile_lanes,
+                             *(self.arrays[k].ptr for k in ("chunks", "runIds", "links")))
 
     def gather_scatter(self, q, k=1, op=_lib.ADD):
         check(libspx().spx_ogs_gather_scatter_srb(ogs_type(q.dtype), op, k, C.byref(self.desc), q.ptr),
@@ -175,7 +176,7 @@ class Ogs:
 
     def set_gather_format(self, gather_sorted=None, scatter_sorted=None):
         """T-CSR gather (Trans) and scatter (NoTrans) on the sorted-run blocks (True) or the T-CSR row
-        blocks (False); None keeps the current choice (defaults: gather CSR, scatter sorted)."""
+        blocks (False); None keeps the current choice (both sorted by default)."""
         enc = lambda x: -1 if x is None else (1 if x else 0)
         checkh(libspxhost().spxh_ogs_set_gather_format(self.h, enc(gather_sorted), enc(scatter_sorted)),
                "spxh_ogs_set_gather_format")

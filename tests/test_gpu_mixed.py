@@ -67,7 +67,7 @@ def test_gs_mixed_block_caps(setups, oracle, cap, st):
         device.check_kernel_errors()
         assert np.array_equal(dq.to_numpy(), oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, q))
     finally:
-        g.set_block_nnz(2048, gs_cap=512)
+        g.set_block_nnz(2048, gs_cap=2048)
 
 
 def test_gs_mixed_f64_is_plain(setups, oracle):

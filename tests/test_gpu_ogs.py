@@ -144,7 +144,7 @@ def test_result_independent_of_row_blocks(setups, oracle, cap, variant):
         device.check_kernel_errors()
         assert same(dout.to_numpy(), o.scatter(gq_want, np.zeros(g.N), 1, spx.NOTRANS))
     finally:
-        g.set_block_nnz(2048, 512)
+        g.set_block_nnz(2048, 2048)
         g.set_gs_format(True)
         device.set_tuning("ogs_gs", variant=-1)
 
@@ -175,7 +175,7 @@ def test_srb_equals_csr_format(setups, oracle, mesh, dtype, variant, cap, xcd):
                 assert same(res[0], o.gather_scatter(q, k, op))
     finally:
         g.set_gs_format(True, -1)
-        g.set_block_nnz(2048, gs_cap=512)
+        g.set_block_nnz(2048, gs_cap=2048)
         device.set_tuning("ogs_gs_srb", variant=-1)
 
 
@@ -216,8 +216,8 @@ def test_srb_gather_scatter_equal_csr(setups, oracle, mesh, dtype, lanes, cap):
             assert same(outs[0], outs[1])
             assert same(outs[0], o.scatter(gsrc, base, k, spx.NOTRANS))
     finally:
-        g.set_gather_format(False, True)
-        g.set_block_nnz(2048, gs_cap=512)
+        g.set_gather_format(True, True)
+        g.set_block_nnz(2048, gs_cap=2048)
         device.set_tuning("ogs_gs_srb", variant=-1)
 
 

@@ -86,22 +86,28 @@ def test_row_larger_than_cap_is_refused():
 # ---------------------------------------------------------------------------
 
 def decode_srb(s):
-    """Local index and previous-member position of every block position, by the device kernel's rule."""
-    ids = np.zeros(s["nnz"], np.int64)
-    prev = np.full(s["nnz"], -1, np.int64)
+    """(local index, previous-member position, row if row-leading) of every entry of every tile, by
+    the device kernels' decoding rule, tile by tile in position order."""
+    cpt = s["chunks_per_tile"]
+    tiles = []
     for b in range(s["nblocks"]):
-        e0, e1 = s["entryStarts"][b], s["entryStarts"][b + 1]
-        c0 = s["chunkStarts"][b]
-        for pos in range(e1 - e0):
-            c, l = c0 + (pos >> 6), pos & 63
-            rm, lm = int(s["runMask"][c]), int(s["linkMask"][c])
-            below = rm & ((2 << l) - 1)
-            k = bin(below).count("1") - 1
-            top = below.bit_length() - 1
-            ids[e0 + pos] = s["runIds"][s["runBase"][c] + k] + (l - top)
-            if lm >> l & 1:
-                prev[e0 + pos] = s["links"][s["linkBase"][c] + bin(lm & ((1 << l) - 1)).count("1")]
-    return ids, prev
+        ent = []
+        for k in range(cpt):
+            C = s["chunks"][b * cpt + k]
+            rm, lm = int(C["runMask"]), int(C["linkMask"])
+            for l in range(int(C["nvalid"])):
+                below = rm & ((2 << l) - 1)
+                run = bin(below).count("1") - 1
+                top = below.bit_length() - 1
+                idx = int(s["runIds"][C["runBase"] + run]) + (l - top)
+                before = bin(lm & ((1 << l) - 1)).count("1")
+                prev = int(s["links"][C["linkBase"] + before]) if lm >> l & 1 else -1
+                row = int(C["chunkRow"]) + (l - before) if prev < 0 else None
+                ent.append((idx, prev, row))
+            # no bits past the entries (a padding record keeps run bit 0, so it still decodes)
+            assert all(not (rm >> l & 1) and not (lm >> l & 1) for l in range(max(int(C["nvalid"]), 1), 64))
+        tiles.append(ent)
+    return tiles
 
 
 @pytest.mark.parametrize("mesh,cap", [((2, 2, 2, 2), 64), ((3, 2, 1, 3), 100), ((4, 3, 2, 4), 4096),
@@ -112,44 +118,34 @@ def test_srb_layout_matches_csr(mesh, cap):
     rs, ci = c["rowStartsT"], c["colIdsT"]
     bs = ogs.block_rows(rs, cap)
     s = ogs.srb_build(rs, ci, bs, cap)
-    assert s["nblocks"] == bs.size - 1 and s["nnz"] == ci.size
+    cpt = s["chunks_per_tile"]
+    assert cpt >= 4 and cpt & (cpt - 1) == 0 and 64 * cpt >= cap          # power of two, covers the cap
+    assert s["nblocks"] == bs.size - 1 and s["nnz"] == ci.size and s["nchunks"] == s["nblocks"] * cpt
     assert np.array_equal(s["entryStarts"], rs[bs])
     assert s["nlinks"] == ci.size - (rs.size - 1)            # every member but the first of each row
-    dec, prev = decode_srb(s)
-    for b in range(bs.size - 1):
+    for b, ent in enumerate(decode_srb(s)):
         e0, e1 = rs[bs[b]], rs[bs[b + 1]]
         n = e1 - e0
-        assert s["chunkStarts"][b + 1] - s["chunkStarts"][b] == (n + 63) // 64
-        assert np.array_equal(dec[e0:e1], np.sort(ci[e0:e1]))          # ascending local index
-        # following the links from each row's first member walks the CSR row in order
-        pos_of = {int(x): i for i, x in enumerate(dec[e0:e1])}
+        assert len(ent) == n
+        dec = [e[0] for e in ent]
+        assert dec == sorted(int(x) for x in ci[e0:e1])        # ascending local index
+        # the links from each row's first member walk the CSR row in order; rows in CSR order
         nxt = {}
-        for i in range(n):
-            if prev[e0 + i] >= 0:
-                assert prev[e0 + i] < i and int(prev[e0 + i]) not in nxt
-                nxt[int(prev[e0 + i])] = i
-        for r in range(bs[b], bs[b + 1]):
-            row = ci[rs[r]:rs[r + 1]]
-            i = pos_of[int(row[0])]
-            assert prev[e0 + i] == -1
-            walk = [int(dec[e0 + i])]
+        for i, (_, prev, _) in enumerate(ent):
+            if prev >= 0:
+                assert prev < i and prev not in nxt
+                nxt[prev] = i
+        lead = [i for i, e in enumerate(ent) if e[1] < 0]
+        assert [ent[i][2] for i in lead] == list(range(bs[b], bs[b + 1]))
+        for r, i in zip(range(bs[b], bs[b + 1]), lead):
+            walk = [ent[i][0]]
             while i in nxt:
                 i = nxt[i]
-                walk.append(int(dec[e0 + i]))
-            assert walk == [int(x) for x in row]
-    for c in range(s["nchunks"]):
-        assert int(s["runMask"][c]) & 1
-    # row of a row-leading position: chunkRow of its chunk plus the leading positions before it
-    for b in range(bs.size - 1):
-        e0, e1 = rs[bs[b]], rs[bs[b + 1]]
-        lead_rows = []
-        for pos in range(e1 - e0):
-            c = s["chunkStarts"][b] + (pos >> 6)
-            if prev[e0 + pos] < 0:
-                lm = int(s["linkMask"][c])
-                before = (pos & 63) - bin(lm & ((1 << (pos & 63)) - 1)).count("1")
-                lead_rows.append(int(s["chunkRow"][c]) + before)
-        assert lead_rows == list(range(bs[b], bs[b + 1]))
+                walk.append(ent[i][0])
+            assert walk == [int(x) for x in ci[rs[r]:rs[r + 1]]]
+    for rec in s["chunks"][: s["nchunks"]]:
+        assert int(rec["runMask"]) & 1                       # every record decodes (padding included)
+        assert 0 <= rec["runBase"] < max(s["nruns"], 1)
 
 
 def test_srb_rejects_bad_blocks():
