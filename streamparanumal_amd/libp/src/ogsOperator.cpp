@@ -57,9 +57,11 @@ ogsOperator_t::ogsOperator_t(platform_t &platform_, Kind kind_, const dlong Nrow
   SetGsBlockNnz(gsBlockNnz_);
   auto localRows = [&](dlong nrows, const memory<dlong> &rs, const memory<dlong> &ci) {
     memory<dlong> m(Ncols > 0 ? Ncols : 1);
-    std::fill(m.ptr(), m.ptr() + Ncols, -1);
+#pragma omp parallel for schedule(static)
+    for (dlong i = 0; i < Ncols; ++i) m[i] = -1;
+#pragma omp parallel for schedule(static)
     for (dlong r = 0; r < nrows; ++r)
-      for (dlong i = rs[r]; i < rs[r + 1]; ++i) m[ci[i]] = r;
+      for (dlong i = rs[r]; i < rs[r + 1]; ++i) m[ci[i]] = r;  // each local node in at most one row
     return platform.malloc<dlong>(Ncols, m);
   };
   o_localRowT = localRows(NrowsT, rowStartsT, colIdsT);

@@ -22,6 +22,10 @@
 #include <cstdlib>
 #include <numeric>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include "ogs.hpp"
 
 namespace libp {
@@ -45,6 +49,38 @@ void lcgInts(int64_t n, uint64_t seed, std::vector<int32_t> &out) {
 
 }  // namespace
 
+// Parallel exclusive prefix sum of v[0..n) in place; returns the total.
+template <typename V>
+static V ExclusiveScan(V *v, dlong n) {
+  if (n <= 0) return 0;
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  std::vector<V> part(nt + 1, 0);
+#pragma omp parallel num_threads(nt)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    const dlong lo = (dlong)((int64_t)n * t / nt), hi = (dlong)((int64_t)n * (t + 1) / nt);
+    V sum = 0;
+    for (dlong i = lo; i < hi; ++i) sum += v[i];
+    part[t + 1] = sum;
+#pragma omp barrier
+#pragma omp single
+    for (int k = 0; k < nt; ++k) part[k + 1] += part[k];
+    V run = part[t];
+    for (dlong i = lo; i < hi; ++i) {
+      const V x = v[i];
+      v[i] = run;
+      run += x;
+    }
+  }
+  return part[nt];
+}
+
 LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const bool unique, const int rank) {
   LIBP_ABORT("Invalid ogs setup requested", (kind == Unsigned && unique) || (kind == Halo && unique));
   LIBP_ABORT("ogs::Halo setups describe exchanges between ranks; this build runs one rank per GPU "
@@ -55,6 +91,7 @@ LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const 
   // ---- nonzero positions and |id| range
   hlong maxAbs = 0;
   dlong Nids = 0;
+#pragma omp parallel for reduction(max : maxAbs) reduction(+ : Nids)
   for (dlong i = 0; i < N; ++i) {
     const hlong a = std::llabs(ids[i]);
     if (a) {
@@ -77,21 +114,35 @@ LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const 
   dlong ngroups = 0;
   const bool dense = maxAbs < (hlong)INT32_MAX && maxAbs <= 4 * (hlong)Nids + (1 << 20);
   if (dense) {
-    std::vector<dlong> first(maxAbs + 1, -1);
+    // first[a] = smallest k with |id| = a (atomic minimum), then groups are
+    // numbered by a prefix count of the first occurrences in k order
+    std::vector<dlong> first(maxAbs + 1, std::numeric_limits<dlong>::max());
+#pragma omp parallel for schedule(static)
     for (dlong k = 0; k < Nids; ++k) {
       const hlong a = std::llabs(ids[posOf(k)]);
-      dlong &f = first[a];
-      if (f < 0) f = ngroups++;
-      grp[k] = f;
+      dlong cur = __atomic_load_n(&first[a], __ATOMIC_RELAXED);
+      while (k < cur && !__atomic_compare_exchange_n(&first[a], &cur, k, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+      }
     }
+    std::vector<dlong> gnum(Nids);
+#pragma omp parallel for schedule(static)
+    for (dlong k = 0; k < Nids; ++k) gnum[k] = (first[std::llabs(ids[posOf(k)])] == k) ? 1 : 0;
+    ngroups = ExclusiveScan(gnum.data(), Nids);
+#pragma omp parallel for schedule(static)
+    for (dlong k = 0; k < Nids; ++k) grp[k] = gnum[first[std::llabs(ids[posOf(k)])]];
     if (unique) {
       rankOf.assign(ngroups, 0);
-      dlong r = 0;
+      std::vector<dlong> present(maxAbs + 1);
+#pragma omp parallel for schedule(static)
+      for (hlong a = 0; a <= maxAbs; ++a) present[a] = (a > 0 && first[a] != std::numeric_limits<dlong>::max()) ? 1 : 0;
+      ExclusiveScan(present.data(), (dlong)(maxAbs + 1));
+#pragma omp parallel for schedule(static)
       for (hlong a = 1; a <= maxAbs; ++a)
-        if (first[a] >= 0) rankOf[first[a]] = r++;
+        if (first[a] != std::numeric_limits<dlong>::max()) rankOf[gnum[first[a]]] = present[a];
     }
   } else {
     std::vector<std::pair<hlong, dlong>> kv(Nids);
+#pragma omp parallel for schedule(static)
     for (dlong k = 0; k < Nids; ++k) kv[k] = {std::llabs(ids[posOf(k)]), k};
     std::sort(kv.begin(), kv.end());
     // groups in increasing |id|; first member (smallest k) opens the group
@@ -107,6 +158,7 @@ LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const 
     std::sort(order.begin(), order.end(), [&](dlong x, dlong y) { return groupFirst[x] < groupFirst[y]; });
     std::vector<dlong> fa(ngroups);  // |id| rank -> first-appearance index
     for (dlong j = 0; j < ngroups; ++j) fa[order[j]] = j;
+#pragma omp parallel for schedule(static)
     for (dlong k = 0; k < Nids; ++k) grp[k] = fa[sortedGroupOf[k]];
     if (unique) {
       rankOf.assign(ngroups, 0);
@@ -114,14 +166,19 @@ LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const 
     }
   }
 
-  // ---- members of each group, in increasing local index (counting sort)
+  // ---- members of each group, in increasing local index: atomic placement,
+  // then each group's few members sorted
   std::vector<dlong> gStart(ngroups + 1, 0);
-  for (dlong k = 0; k < Nids; ++k) ++gStart[grp[k] + 1];
-  for (dlong g = 0; g < ngroups; ++g) gStart[g + 1] += gStart[g];
+#pragma omp parallel for schedule(static)
+  for (dlong k = 0; k < Nids; ++k) __atomic_fetch_add(&gStart[grp[k]], (dlong)1, __ATOMIC_RELAXED);
+  ExclusiveScan(gStart.data(), ngroups + 1);
   std::vector<dlong> members(Nids);
   {
     std::vector<dlong> cur(gStart.begin(), gStart.end() - 1);
-    for (dlong k = 0; k < Nids; ++k) members[cur[grp[k]]++] = posOf(k);
+#pragma omp parallel for schedule(static)
+    for (dlong k = 0; k < Nids; ++k) members[__atomic_fetch_add(&cur[grp[k]], (dlong)1, __ATOMIC_RELAXED)] = posOf(k);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (dlong g = 0; g < ngroups; ++g) std::sort(members.begin() + gStart[g], members.begin() + gStart[g + 1]);
   }
 
   // ---- signs: unique flagging or the caller's flags
@@ -130,6 +187,7 @@ LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const 
   if (unique) {
     std::vector<int32_t> rands;
     lcgInts(ngroups, (uint64_t)rank, rands);
+#pragma omp parallel for schedule(static)
     for (dlong g = 0; g < ngroups; ++g) {
       const dlong a = gStart[g], e = gStart[g + 1];
       const dlong m = (dlong)((uint32_t)rands[rankOf[g]] % (uint32_t)(e - a));
@@ -140,32 +198,41 @@ LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const 
       hasPos[g] = 1;
     }
   } else {
+    int undefined = 0;
+#pragma omp parallel for schedule(static) reduction(| : undefined)
     for (dlong g = 0; g < ngroups; ++g) {
       int npos = 0;
       for (dlong i = gStart[g]; i < gStart[g + 1]; ++i) npos += (ids[members[i]] > 0);
       hasPos[g] = npos > 0;
-      if (npos != 1) gather_defined = false;
+      if (npos != 1) undefined = 1;
     }
+    gather_defined = !undefined;
   }
 
   // ---- rows: positive groups first, each class in first-appearance order
   std::vector<dlong> rowOf(ngroups);
   dlong NP = 0;
-  for (dlong g = 0; g < ngroups; ++g)
-    if (hasPos[g]) rowOf[g] = NP++;
   {
-    dlong r = NP;
-    for (dlong g = 0; g < ngroups; ++g)
-      if (!hasPos[g]) rowOf[g] = r++;
+    std::vector<dlong> posRank(ngroups), negRank(ngroups);
+#pragma omp parallel for schedule(static)
+    for (dlong g = 0; g < ngroups; ++g) {
+      posRank[g] = hasPos[g] ? 1 : 0;
+      negRank[g] = hasPos[g] ? 0 : 1;
+    }
+    NP = ExclusiveScan(posRank.data(), ngroups);
+    ExclusiveScan(negRank.data(), ngroups);
+#pragma omp parallel for schedule(static)
+    for (dlong g = 0; g < ngroups; ++g) rowOf[g] = hasPos[g] ? posRank[g] : NP + negRank[g];
   }
 
   // ---- T-CSR (every member) and N-CSR (positive members)
   memory<dlong> rowStartsT(ngroups + 1), colIdsT(Nids);
-  rowStartsT[0] = 0;
   {
-    std::vector<dlong> sizeOfRow(ngroups);
-    for (dlong g = 0; g < ngroups; ++g) sizeOfRow[rowOf[g]] = gStart[g + 1] - gStart[g];
-    for (dlong r = 0; r < ngroups; ++r) rowStartsT[r + 1] = rowStartsT[r] + sizeOfRow[r];
+#pragma omp parallel for schedule(static)
+    for (dlong g = 0; g < ngroups; ++g) rowStartsT[rowOf[g]] = gStart[g + 1] - gStart[g];
+    rowStartsT[ngroups] = 0;
+    ExclusiveScan(rowStartsT.ptr(), ngroups + 1);
+#pragma omp parallel for schedule(static)
     for (dlong g = 0; g < ngroups; ++g)
       std::copy(members.begin() + gStart[g], members.begin() + gStart[g + 1], colIdsT.ptr() + rowStartsT[rowOf[g]]);
   }
@@ -173,18 +240,21 @@ LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const 
   dlong nnzN = 0;
   if (kind == Signed) {
     rowStartsN.malloc(NP + 1);
-    rowStartsN[0] = 0;
+#pragma omp parallel for schedule(static)
     for (dlong r = 0; r < NP; ++r) {
       dlong c = 0;
       for (dlong i = rowStartsT[r]; i < rowStartsT[r + 1]; ++i) c += (ids[colIdsT[i]] > 0);
-      rowStartsN[r + 1] = rowStartsN[r] + c;
+      rowStartsN[r] = c;
     }
-    nnzN = rowStartsN[NP];
+    rowStartsN[NP] = 0;
+    nnzN = ExclusiveScan(rowStartsN.ptr(), NP + 1);
     colIdsN.malloc(nnzN);
-    dlong c = 0;
-    for (dlong r = 0; r < NP; ++r)
+#pragma omp parallel for schedule(static)
+    for (dlong r = 0; r < NP; ++r) {
+      dlong c = rowStartsN[r];
       for (dlong i = rowStartsT[r]; i < rowStartsT[r + 1]; ++i)
         if (ids[colIdsT[i]] > 0) colIdsN[c++] = colIdsT[i];
+    }
   } else {
     rowStartsN = rowStartsT;
     colIdsN = colIdsT;
@@ -264,9 +334,13 @@ void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform
   rsN[0] = 0;
   for (const ogsBase_t *p : parts) {
     const ogsOperator_t &op = *p->gatherLocal;
+#pragma omp parallel for schedule(static)
     for (dlong r = 0; r < op.NrowsT; ++r) rsT[rowOffT + r + 1] = eOffT + op.rowStartsT[r + 1];
+#pragma omp parallel for schedule(static)
     for (dlong e = 0; e < op.nnzT; ++e) ciT[eOffT + e] = nodeOff + op.colIdsT[e];
+#pragma omp parallel for schedule(static)
     for (dlong r = 0; r < op.NrowsN; ++r) rsN[rowOffN + r + 1] = eOffN + op.rowStartsN[r + 1];
+#pragma omp parallel for schedule(static)
     for (dlong e = 0; e < op.nnzN; ++e) ciN[eOffN + e] = nodeOff + op.colIdsN[e];
     nodeOff += p->N;
     rowOffT += op.NrowsT;
