@@ -166,9 +166,7 @@ def degree_sweep(K, W, seed=2, target=1e8):
     rows = []
     for p in range(1, 16):
         n = int(round((target / (p + 1) ** 3) ** (1.0 / 3.0)))
-        ids = ogs.hex_ids(n, n, n, p)
-        g = ogs.Ogs(ids)
-        del ids
+        g = ogs.Ogs.hex(n, n, n, p, keep_ids=False)
         Nl, Ng = g.N, g.NgatherGlobal
         q = DeviceArray(Nl)
         device.fill_uniform(q, seed, 0)
@@ -221,9 +219,7 @@ def main():
     # ---------------------------------------------------------------- BS6-BS8
     nx, p = args.nx, args.degree
     t0 = time.perf_counter()
-    ids = ogs.hex_ids(nx, nx, nx, p)
-    g = ogs.Ogs(ids)
-    del ids
+    g = ogs.Ogs.hex(nx, nx, nx, p, keep_ids=False)
     Nl, Ng = g.N, g.NgatherGlobal
     log(f"mesh {nx}^3 p={p}: Nl={Nl} Ng={Ng}, setup {time.perf_counter() - t0:.1f}s "
         f"(ogs {g.info['setup_seconds']:.1f}s), blocks {g.info['NblocksT']} x <= {g.info['block_nnz']}")

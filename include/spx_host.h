@@ -48,6 +48,14 @@ int spxh_platform_open(int device_id);
 int spxh_ogs_setup(int32_t n, int64_t *ids, int kind, int unique, int verbose, spxh_ogs_t *out);
 int spxh_ogs_destroy(spxh_ogs_t ogs);
 int spxh_ogs_info(spxh_ogs_t ogs, spxh_ogs_info_t *info);
+/* ogs_t set up for the lexicographic GLL ids of an nx x ny x nz hex box at
+   degree p (spxh_mesh_hex_ids) without the ids: the same operator and flags
+   as spxh_ogs_setup over those ids, in closed form (meshSetupBoxHex3D.cpp
+   and ogsSetup.cpp:87-290 give the operator this reproduces).  ids
+   (optional, nx*ny*nz*(p+1)^3 entries) receives the ids as spxh_ogs_setup
+   would leave them. */
+int spxh_ogs_setup_hex(int32_t nx, int32_t ny, int32_t nz, int p, int kind, int unique, int64_t *ids,
+                       spxh_ogs_t *out);
 /* The block-diagonal union of nparts set-up handles as one new handle (no
    reference counterpart): part i's local nodes follow parts 0..i-1's, so a
    batch of independent meshes is gathered in one launch.  The parts stay
@@ -111,6 +119,9 @@ int spxh_ogs_gather_scatter_mixed(spxh_ogs_t ogs, int storage, int k, void *v);
    flagging (written back into ids), row numbering, T- and N-CSR. */
 typedef struct spxh_csr_s *spxh_csr_t;
 int spxh_csr_build(int32_t n, int64_t *ids, int kind, int unique, spxh_csr_t *out);
+/* the same for a hex box in closed form (see spxh_ogs_setup_hex) */
+int spxh_csr_build_hex(int32_t nx, int32_t ny, int32_t nz, int p, int kind, int unique, int64_t *ids,
+                       spxh_csr_t *out);
 /* sizes[0..4] = NrowsT, NrowsN, nnzT, nnzN, gather_defined */
 int spxh_csr_sizes(spxh_csr_t csr, int32_t *sizes);
 int spxh_csr_copy(spxh_csr_t csr, int which, int32_t *rowStarts, int32_t *colIds);

@@ -9,7 +9,7 @@ import numpy as np
 
 from . import _lib
 from .device import DeviceArray, Graph, convert, fill_uniform
-from .ogs import Ogs, hex_ids
+from .ogs import Ogs
 from .streaming import CgPlan, Reducer
 
 _A = 6364136223846793005
@@ -49,11 +49,7 @@ class Batch:
         self.dtype = np.dtype(dtype)
         self.storage = _lib.storage_of(self.dtype)
         if parts is None:
-            parts = []
-            for (n, p) in self.meshes:
-                ids = hex_ids(n, n, n, p)
-                parts.append(Ogs(ids))
-                del ids
+            parts = [Ogs.hex(n, n, n, p, keep_ids=False) for (n, p) in self.meshes]
         self.parts = parts
         self.union = union if union is not None else Ogs.batch(self.parts)
         self.offsets = np.zeros(len(self.parts) + 1, dtype=np.int64)

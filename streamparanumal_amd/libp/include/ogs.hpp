@@ -79,6 +79,12 @@ class ogsBase_t {
   // rows follow theirs, so one launch gathers every part.  All parts share
   // one Kind; the total must fit dlong.
   void SetupBatch(const std::vector<const ogsBase_t *> &parts, platform_t &_platform);
+  // The operator Setup builds from the lexicographic GLL ids of an
+  // nx x ny x nz hex box at degree p (mesh_t::HexBoxGlobalIds), in closed
+  // form without the ids; idsOut (optional, nx*ny*nz*(p+1)^3 entries)
+  // receives the ids as Setup leaves them (sign-flagged with unique=true).
+  void SetupHexBox(const dlong nx, const dlong ny, const dlong nz, const int p, comm_t _comm, const Kind _kind,
+                   const bool _unique, platform_t &_platform, hlong *idsOut = nullptr);
 
   // the local gather operator (CSR on the device); null before Setup
   std::shared_ptr<ogsOperator_t> localOperator() const { return gatherLocal; }
@@ -87,6 +93,7 @@ class ogsBase_t {
  protected:
   std::shared_ptr<ogsOperator_t> gatherLocal;
   void AssertGatherDefined();
+  void SetupFromCSR(LocalCSR &csr);
   double setupSeconds_ = 0.0;
 };
 

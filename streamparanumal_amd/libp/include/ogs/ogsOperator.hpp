@@ -161,6 +161,12 @@ struct LocalCSR {
 // (written back into ids), row numbering and both CSRs.
 LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const bool unique, const int rank);
 
+// The same for the lexicographic GLL ids of an nx x ny x nz hex box at degree
+// p, in closed form (no ids needed); idsOut, when given, receives the ids as
+// BuildLocalCSR would leave them.
+LocalCSR BuildHexBoxCSR(const dlong nx, const dlong ny, const dlong nz, const int p, const Kind kind,
+                        const bool unique, const int rank, hlong *idsOut);
+
 // Greedy row blocking: consecutive rows, at most `cap` entries per block.
 // Aborts when one row alone exceeds cap (ogsOperator.cpp:535-536).
 void blockRowsGreedy(const dlong Nrows, const memory<dlong> rowStarts, const int cap, dlong &Nblocks,

@@ -81,6 +81,20 @@ int spxh_ogs_setup(int32_t n, int64_t *ids, int kind, int unique, int verbose, s
   });
 }
 
+int spxh_ogs_setup_hex(int32_t nx, int32_t ny, int32_t nz, int p, int kind, int unique, int64_t *ids,
+                       spxh_ogs_t *out) {
+  if (!out) return err(SPX_ERR_ARG, "spxh_ogs_setup_hex: null out");
+  if (!state().open) return err(SPX_ERR_NODEVICE, "spxh_ogs_setup_hex: call spxh_platform_open first");
+  if (kind < 0 || kind > 2) return err(SPX_ERR_ARG, "spxh_ogs_setup_hex: kind must be 0, 1 or 2");
+  *out = nullptr;
+  return guarded([&] {
+    auto h = std::make_unique<spxh_ogs_s>();
+    h->ogs.SetupHexBox(nx, ny, nz, p, comm_t::world(), (ogs::Kind)kind, unique != 0, state().platform,
+                       reinterpret_cast<hlong *>(ids));
+    *out = h.release();
+  });
+}
+
 int spxh_ogs_setup_batch(const spxh_ogs_t *parts, int32_t nparts, spxh_ogs_t *out) {
   if (!out || !parts || nparts < 1) return err(SPX_ERR_ARG, "spxh_ogs_setup_batch: bad arguments");
   if (!state().open) return err(SPX_ERR_NODEVICE, "spxh_ogs_setup_batch: call spxh_platform_open first");
@@ -304,6 +318,18 @@ int spxh_csr_build(int32_t n, int64_t *ids, int kind, int unique, spxh_csr_t *ou
     if (n) std::copy(ids, ids + n, m.ptr());
     h->csr = ogs::BuildLocalCSR(n, m, (ogs::Kind)kind, unique != 0, 0);
     if (n) std::copy(m.ptr(), m.ptr() + n, ids);
+    *out = h.release();
+  });
+}
+
+int spxh_csr_build_hex(int32_t nx, int32_t ny, int32_t nz, int p, int kind, int unique, int64_t *ids,
+                       spxh_csr_t *out) {
+  if (!out) return err(SPX_ERR_ARG, "spxh_csr_build_hex: null out");
+  if (kind < 0 || kind > 2) return err(SPX_ERR_ARG, "spxh_csr_build_hex: bad kind");
+  *out = nullptr;
+  return guarded([&] {
+    auto h = std::make_unique<spxh_csr_s>();
+    h->csr = ogs::BuildHexBoxCSR(nx, ny, nz, p, (ogs::Kind)kind, unique != 0, 0, reinterpret_cast<hlong *>(ids));
     *out = h.release();
   });
 }

@@ -81,10 +81,15 @@ void mesh_t::SetupBoxHex3D() {
   HexBoxGlobalIds(nx, ny, nz, N, globalIds.ptr());
 }
 
+// The box's operator is built in closed form (ogs_t::SetupHexBox): the same
+// operator, and the same sign-flagged ids, as ogs.Setup over globalIds.
 void mesh_t::GatherScatterSetup() {
   const bool verbose = settings.compareSetting("VERBOSE", "TRUE");
   const bool unique = true;  // flag a unique node in every gather node
-  ogs.Setup(Nelements * Np, globalIds, comm, ogs::Signed, ogs::Auto, unique, verbose, platform);
+  ogs.SetupHexBox(nx, ny, nz, N, comm, ogs::Signed, unique, platform, globalIds.ptr());
+  if (verbose && comm.rank() == 0)
+    std::cout << "ogs Setup: " << ogs.NlocalT << " gather nodes; setup " << ogs.setupSeconds() << " seconds."
+              << std::endl;
   GlobalToLocal.malloc((size_t)Nelements * Np);
   ogs.SetupGlobalToLocalMapping(GlobalToLocal);
   o_GlobalToLocal = platform.malloc(GlobalToLocal);

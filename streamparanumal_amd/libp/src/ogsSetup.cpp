@@ -286,6 +286,28 @@ void ogsBase_t::Setup(const dlong _N, memory<hlong> ids, comm_t _comm, const Kin
   unique = _unique;
 
   LocalCSR csr = BuildLocalCSR(N, ids, kind, unique, comm.rank());
+  SetupFromCSR(csr);
+  setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (verbose && comm.rank() == 0)
+    std::cout << "ogs Setup: " << NlocalT << " gather nodes, " << gatherLocal->nnzT << " entries; setup "
+              << setupSeconds_ << " seconds." << std::endl;
+}
+
+void ogsBase_t::SetupHexBox(const dlong nx, const dlong ny, const dlong nz, const int p, comm_t _comm,
+                            const Kind _kind, const bool _unique, platform_t &_platform, hlong *idsOut) {
+  Free();
+  const auto t0 = std::chrono::steady_clock::now();
+  platform = _platform;
+  comm = _comm;
+  kind = _kind;
+  unique = _unique;
+  LocalCSR csr = BuildHexBoxCSR(nx, ny, nz, p, kind, unique, comm.rank(), idsOut);
+  N = csr.nnzT;
+  SetupFromCSR(csr);
+  setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void ogsBase_t::SetupFromCSR(LocalCSR &csr) {
   gather_defined = csr.gather_defined;
 
   NlocalT = csr.NrowsT;
@@ -299,11 +321,6 @@ void ogsBase_t::Setup(const dlong _N, memory<hlong> ids, comm_t _comm, const Kin
   gatherLocal = std::make_shared<ogsOperator_t>(platform, kind == Unsigned ? Unsigned : Signed, csr.NrowsN,
                                                 csr.NrowsT, N, csr.rowStartsT, csr.colIdsT, csr.rowStartsN,
                                                 csr.colIdsN);
-
-  setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (verbose && comm.rank() == 0)
-    std::cout << "ogs Setup: " << NlocalT << " gather nodes, " << gatherLocal->nnzT << " entries; setup "
-              << setupSeconds_ << " seconds." << std::endl;
 }
 
 void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform_t &_platform) {

@@ -41,6 +41,19 @@ def build_csr(ids, kind=_lib.SIGNED, unique=True):
     h = C.c_void_p()
     checkh(libspxhost().spxh_csr_build(ids.size, ids.ctypes.data, kind, 1 if unique else 0, C.byref(h)),
            "spxh_csr_build")
+    return ids, _read_csr(h)
+
+
+def build_csr_hex(nx, ny, nz, p, kind=_lib.SIGNED, unique=True):
+    """The same for the hex box's lexicographic ids, in closed form: (ids_after, dict of CSR arrays)."""
+    ids = np.empty(nx * ny * nz * (p + 1) ** 3, dtype=np.int64)
+    h = C.c_void_p()
+    checkh(libspxhost().spxh_csr_build_hex(nx, ny, nz, p, kind, 1 if unique else 0, ids.ctypes.data,
+                                           C.byref(h)), "spxh_csr_build_hex")
+    return ids, _read_csr(h)
+
+
+def _read_csr(h):
     try:
         sz = np.zeros(5, dtype=np.int32)
         checkh(libspxhost().spxh_csr_sizes(h, sz.ctypes.data), "spxh_csr_sizes")
@@ -52,7 +65,7 @@ def build_csr(ids, kind=_lib.SIGNED, unique=True):
             checkh(libspxhost().spxh_csr_copy(h, which, rs.ctypes.data, ci.ctypes.data), "spxh_csr_copy")
             out["rowStarts" + tag] = rs
             out["colIds" + tag] = ci[:nnz]
-        return ids, out
+        return out
     finally:
         libspxhost().spxh_csr_destroy(h)
 
@@ -130,6 +143,21 @@ class Ogs:
                                            1 if verbose else 0, C.byref(self.h)), "spxh_ogs_setup")
         self.ids = ids  # sign-flagged when unique
         self.refresh()
+
+    @classmethod
+    def hex(cls, nx, ny, nz, p, kind=_lib.SIGNED, unique=True, keep_ids=True, device_id=0):
+        """ogs_t of the hex box's lexicographic GLL ids, built in closed form (the same operator and
+        flags as Ogs(hex_ids(nx, ny, nz, p)))."""
+        platform_open(device_id)
+        self = cls.__new__(cls)
+        ids = np.empty(nx * ny * nz * (p + 1) ** 3, dtype=np.int64) if keep_ids else None
+        self.h = C.c_void_p()
+        checkh(libspxhost().spxh_ogs_setup_hex(nx, ny, nz, p, kind, 1 if unique else 0,
+                                               ids.ctypes.data if keep_ids else None, C.byref(self.h)),
+               "spxh_ogs_setup_hex")
+        self.ids = ids
+        self.refresh()
+        return self
 
     @classmethod
     def batch(cls, parts):

@@ -62,6 +62,24 @@ def test_setup_matches_oracle(setups, mesh):
 
 
 @pytest.mark.parametrize("mesh", MESHES)
+def test_hex_setup_equals_generic_on_device(setups, oracle, mesh):
+    """ogs_t built in closed form for the box: same CSRs, flags and gather+scatter bits."""
+    say("hex setup", mesh)
+    g, o, ids = setups[mesh]
+    h = ogs.Ogs.hex(*mesh)
+    assert h.info["N"] == g.info["N"] and h.NlocalT == g.NlocalT and h.NlocalP == g.NlocalP
+    assert same(h.ids, g.ids)
+    for which in ("T", "N"):
+        for a, b in zip(h.csr(which), g.csr(which)):
+            assert same(a, b)
+    q = oracle.uniform(g.N, 5)
+    dq = DeviceArray.from_numpy(q)
+    h.gather_scatter(dq)
+    device.check_kernel_errors()
+    assert same(dq.to_numpy(), o.gather_scatter(q))
+
+
+@pytest.mark.parametrize("mesh", MESHES)
 def test_bs8_gather_scatter_double(setups, oracle, mesh):
     say("bs8 gs double", mesh)
     g, o, _ = setups[mesh]

@@ -178,3 +178,31 @@ def test_batch_instances_follow_the_lcg():
                         (4 + int(u[6] * 45), (3, 5, 7, 9)[int(u[7] * 4)])]
     # every degree and both ends of the range occur over the 64 draws
     assert {p for _, p in inst} == {3, 5, 7, 9}
+
+
+# ---------------------------------------------------------------------------
+# closed-form hex-box operator: identical to the generic setup over the ids
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("box", [(1, 1, 1, 1), (2, 1, 1, 1), (2, 3, 4, 2), (3, 2, 2, 5), (1, 4, 2, 7),
+                                 (5, 5, 5, 1), (2, 2, 2, 15), (6, 5, 4, 3)])
+@pytest.mark.parametrize("kind,unique", [(spx.SIGNED, True), (spx.SIGNED, False), (spx.UNSIGNED, False)])
+def test_hex_box_setup_equals_generic(box, kind, unique):
+    ids0, c0 = ogs.build_csr(ogs.hex_ids(*box), kind, unique)
+    ids1, c1 = ogs.build_csr_hex(*box, kind, unique)
+    assert np.array_equal(ids0, ids1)                      # the same flags on the same nodes
+    assert c0.keys() == c1.keys()
+    for k in c0:
+        if isinstance(c0[k], np.ndarray):
+            assert np.array_equal(c0[k], c1[k]), k
+        else:
+            assert c0[k] == c1[k], k
+
+
+def test_hex_box_setup_rejects():
+    with pytest.raises(SpxError):
+        ogs.build_csr_hex(0, 1, 1, 3)
+    with pytest.raises(SpxError):
+        ogs.build_csr_hex(1, 1, 1, 0)
+    with pytest.raises(SpxError):
+        ogs.build_csr_hex(2, 2, 2, 3, spx.UNSIGNED, True)    # Unsigned cannot be unique (ogsSetup.cpp:92)
