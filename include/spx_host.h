@@ -147,6 +147,24 @@ int spxh_srb_destroy(spxh_srb_t srb);
    degree p, n = nx*ny*nz*(p+1)^3 entries (meshSetupBoxHex3D.cpp:87-130) */
 int spxh_mesh_hex_ids(int nx, int ny, int nz, int p, int64_t *ids);
 
+/* element types of mesh_t (include/mesh.hpp:116-119) */
+#define SPXH_TRIANGLES 3
+#define SPXH_QUADRILATERALS 4
+#define SPXH_TETRAHEDRA 6
+#define SPXH_HEXAHEDRA 12
+
+/* elements and nodes per element of a box of `type` (meshSetElementType.cpp,
+   meshReferenceNodes*.cpp: Np = (p+1)(p+2)/2, (p+1)^2, (p+1)(p+2)(p+3)/6,
+   (p+1)^3); 2-D types ignore nz (pass 1) */
+int spxh_mesh_box_shape(int type, int nx, int ny, int nz, int p, int64_t *nelements, int *np);
+/* mesh_t box ids for any element type: 1 + the lexicographic index of each
+   node's point on the degree-p lattice of the box, in the reference's element
+   and node order (meshSetupBoxTri2D.cpp:80-113, meshSetupBoxQuad2D.cpp:79-100,
+   meshSetupBoxTet3D.cpp:87-189, meshBasisTri2D.cpp:81-92,
+   meshBasisTet3D.cpp:81-94); the groups are those meshConnectNodes.cpp:32-82
+   forms */
+int spxh_mesh_box_ids(int type, int nx, int ny, int nz, int p, int64_t *ids);
+
 #ifdef __cplusplus
 }
 #endif

@@ -170,6 +170,8 @@ SPXH_FUNCS = {
     "spxh_csr_destroy": (C.c_int, [vp]),
     "spxh_block_rows": (C.c_int, [i32, vp, i32, vp, vp]),
     "spxh_mesh_hex_ids": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+    "spxh_mesh_box_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
+    "spxh_mesh_box_ids": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
 }
 
 

@@ -1,7 +1,9 @@
-// mesh_t for the BS6-BS8 drivers: a structured box of hexahedra at degree N
-// with GLL-grid global node ids (libs/mesh/meshSetupBoxHex3D.cpp:31-131,
-// meshConnectNodes.cpp:32-82, meshGatherScatterSetup.cpp:31-44).  Coordinates
-// and geometric factors never enter the streaming benchmarks and are not built.
+// mesh_t for the BS6-BS8 drivers: a structured box of triangles,
+// quadrilaterals, tetrahedra or hexahedra at degree N with lattice global node
+// ids (libs/mesh/meshSetupBoxTri2D.cpp, meshSetupBoxQuad2D.cpp,
+// meshSetupBoxTet3D.cpp, meshSetupBoxHex3D.cpp, meshConnectNodes.cpp:32-82,
+// meshGatherScatterSetup.cpp:31-44).  Coordinates and geometric factors never
+// enter the streaming benchmarks and are not built.
 #pragma once
 
 #include "ogs.hpp"
@@ -19,7 +21,14 @@ class mesh_t {
   settings_t settings;
   comm_t comm;
 
+  // element types (include/mesh.hpp:116-119)
+  static constexpr int TRIANGLES = 3;
+  static constexpr int QUADRILATERALS = 4;
+  static constexpr int TETRAHEDRA = 6;
+  static constexpr int HEXAHEDRA = 12;
+
   int rank = 0, size = 1;
+  int elementType = HEXAHEDRA;
   int dim = 3;
   int Nverts = 8, Nfaces = 6;
   int N = 0;       // polynomial degree
@@ -29,7 +38,7 @@ class mesh_t {
   dlong nx = 0, ny = 0, nz = 0;
   dlong Nelements = 0;
   hlong Nnodes = 0;        // vertices of the box
-  hlong NglobalNodes = 0;  // GLL grid nodes
+  hlong NglobalNodes = 0;  // nodes of the box's degree-N lattice
 
   memory<hlong> globalIds;
   ogs::ogs_t ogs;
@@ -44,8 +53,17 @@ class mesh_t {
   // (nx*N+1) x (ny*N+1) x (nz*N+1) GLL grid.
   static void HexBoxGlobalIds(dlong nx, dlong ny, dlong nz, int N, hlong *ids);
 
+  // Nodes per element and elements per box of an element type (Np = 0 for an
+  // unknown type); 2-D types ignore nz.
+  static int NodesPerElement(int type, int N);
+  static long long BoxElements(int type, dlong nx, dlong ny, dlong nz);
+  // 1 + the lexicographic index of every local node's point on the box's
+  // degree-N lattice (nx*N+1) x (ny*N+1) [x (nz*N+1)], for any element type.
+  static void BoxGlobalIds(int type, dlong nx, dlong ny, dlong nz, int N, hlong *ids);
+
  private:
-  void SetupBoxHex3D();
+  void SetElementType(const std::string &eType);
+  void SetupBox();
   void GatherScatterSetup();
 };
 

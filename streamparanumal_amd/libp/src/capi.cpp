@@ -438,4 +438,22 @@ int spxh_mesh_hex_ids(int nx, int ny, int nz, int p, int64_t *ids) {
   return guarded([&] { mesh_t::HexBoxGlobalIds(nx, ny, nz, p, (hlong *)ids); });
 }
 
+int spxh_mesh_box_shape(int type, int nx, int ny, int nz, int p, int64_t *nelements, int *np) {
+  if (!nelements || !np) return err(SPX_ERR_ARG, "spxh_mesh_box_shape: null output");
+  if (nx < 1 || ny < 1 || nz < 1 || p < 1 || p > 15) return err(SPX_ERR_ARG, "spxh_mesh_box_shape: bad size");
+  if (mesh_t::NodesPerElement(type, p) == 0)
+    return err(SPX_ERR_ARG, "spxh_mesh_box_shape: type must be 3 (Tri), 4 (Quad), 6 (Tet) or 12 (Hex)");
+  *nelements = mesh_t::BoxElements(type, nx, ny, nz);
+  *np = mesh_t::NodesPerElement(type, p);
+  return SPX_OK;
+}
+
+int spxh_mesh_box_ids(int type, int nx, int ny, int nz, int p, int64_t *ids) {
+  if (!ids) return err(SPX_ERR_ARG, "spxh_mesh_box_ids: null ids");
+  if (nx < 1 || ny < 1 || nz < 1 || p < 1 || p > 15) return err(SPX_ERR_ARG, "spxh_mesh_box_ids: bad size");
+  if (mesh_t::NodesPerElement(type, p) == 0)
+    return err(SPX_ERR_ARG, "spxh_mesh_box_ids: type must be 3 (Tri), 4 (Quad), 6 (Tet) or 12 (Hex)");
+  return guarded([&] { mesh_t::BoxGlobalIds(type, nx, ny, nz, p, (hlong *)ids); });
+}
+
 }  // extern "C"

@@ -35,6 +35,29 @@ def hex_ids(nx, ny, nz, p):
     return ids
 
 
+ELEMENT_TYPES = {"Tri": 3, "Quad": 4, "Tet": 6, "Hex": 12}
+
+
+def box_shape(element, nx, ny, nz, p):
+    """(elements, nodes per element) of a box of `element` ("Tri", "Quad", "Tet", "Hex")."""
+    ne = C.c_int64()
+    npe = C.c_int()
+    checkh(libspxhost().spxh_mesh_box_shape(ELEMENT_TYPES[element], nx, ny, nz, p, C.byref(ne),
+                                            C.byref(npe)), "spxh_mesh_box_shape")
+    return ne.value, npe.value
+
+
+def box_ids(element, nx, ny, nz, p):
+    """Lattice ids of a box of `element` in the reference's element and node order
+    (meshSetupBoxTri2D.cpp, meshSetupBoxQuad2D.cpp, meshSetupBoxTet3D.cpp, meshSetupBoxHex3D.cpp);
+    2-D types ignore nz."""
+    ne, npe = box_shape(element, nx, ny, nz, p)
+    ids = np.empty(ne * npe, dtype=np.int64)
+    checkh(libspxhost().spxh_mesh_box_ids(ELEMENT_TYPES[element], nx, ny, nz, p, ids.ctypes.data),
+           "spxh_mesh_box_ids")
+    return ids
+
+
 def build_csr(ids, kind=_lib.SIGNED, unique=True):
     """Host half of the setup, no device: returns (ids_after, dict of CSR arrays)."""
     ids = np.ascontiguousarray(ids, dtype=np.int64).copy()
