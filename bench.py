@@ -36,9 +36,21 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def timed(fn, steps, warmup):
-    """(host seconds per call, event seconds per call) over `steps` calls after `warmup`."""
+SETTLE_SECONDS = 0.25
+
+
+def timed(fn, steps, warmup, settle=SETTLE_SECONDS):
+    """(host seconds per call, event seconds per call) over `steps` calls after `warmup` calls.
+
+    Before the warm-up calls, `fn` also runs untimed for `settle` seconds of wall time, so the
+    timed calls see the GPU at its sustained clocks rather than ramping up from idle after the
+    host-side setup (a few warm-up calls of a 0.5 ms kernel are over before the clocks settle)."""
     from streamparanumal_amd import device
+    t_end = time.perf_counter() + settle
+    while time.perf_counter() < t_end:
+        for _ in range(8):
+            fn()
+        device.finish()
     for _ in range(warmup):
         fn()
     device.finish()
@@ -188,17 +200,54 @@ def degree_sweep(K, W, seed=2, target=1e8):
     return rows
 
 
+def report(args, K, W, props, nx, p, Nl, Ng, n, bytes_bs8, host8, ev8, measured_peak, cpu, suite):
+    """The one JSON line: BS8 value and roofline, the suite and the CPU baseline."""
+    value = bytes_bs8 / host8 / 1e9
+    achieved = bytes_bs8 / ev8 / 1e9
+    workload = f"BS8 gather+scatter FP64 (Add, Sym, K=1) on a {nx}^3 hex box at p={p} ({Nl} local / {Ng} global nodes)"
+    if n is not None:
+        workload += f"; suite BS1-BS5 at N=2^{args.log2n}"
+    return {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(host8 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: U[-1,1) from the reference LCG (seed 1); lexicographic GLL ids",
+        "config": {"workload": workload,
+                   "nx": nx, "degree": p, "local_nodes": Nl, "global_nodes": Ng, "vector_n": n,
+                   "bytes_per_step": int(bytes_bs8), "GDOFs": round(Ng / host8 / 1e9, 3),
+                   "pct_spec_peak": round(100 * value / SPEC_HBM_GBS, 1),
+                   "pct_measured_peak": round(100 * achieved / measured_peak, 1) if measured_peak else None,
+                   "measured_peak_GBs": round(measured_peak, 1) if measured_peak else None,
+                   "device": props["name"] or f"{props['arch']} ({props['compute_units']} CUs)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": SPEC_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / SPEC_HBM_GBS, 4), "traffic": read_traffic()},
+        "cpu_baseline": cpu,
+        "suite": suite,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=64, help="hex elements per side for BS6-BS8")
     ap.add_argument("--degree", type=int, default=7)
     ap.add_argument("--log2n", type=int, default=27, help="BS1-BS5 vector length 2^log2n")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-mesh batched comparison")
     ap.add_argument("--no-degree-sweep", action="store_true", help="skip the p=1..15 sweep at ~1e8 nodes")
+    ap.add_argument("--only-bs8", action="store_true",
+                    help="time the BS8 step alone (no suite, no CPU baseline): the command the committed "
+                         "rocprofv3 summaries of the BS8 kernel are taken from")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -247,6 +296,11 @@ def main():
         log(name, json.dumps(d))
 
     entry("BS8_gather_scatter", bytes_bs8, Ng, host8, ev8)
+    if args.only_bs8:
+        del q, gq, g
+        print(json.dumps(report(args, K, W, props, nx, p, Nl, Ng, None, bytes_bs8, host8, ev8, None, None, suite)),
+              flush=True)
+        return
     h, e = timed(lambda: g.gather(gq, q, 1, spx.ADD, spx.TRANS), K, W)
     entry("BS6_gather", bytes_bs6, Ng, h, e)
     h, e = timed(lambda: g.scatter(q, gq, 1, spx.NOTRANS), K, W)
@@ -371,39 +425,12 @@ def main():
         suite["batched_64_meshes"] = batched_suite(K, W)
 
     # ---------------------------------------------------------------- report
-    value = bytes_bs8 / host8 / 1e9
-    achieved = bytes_bs8 / ev8 / 1e9
-    traffic = read_traffic()
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread) ...")
         cpu = cpu_baseline()
         log("cpu baseline", json.dumps(cpu))
-    out = {
-        "metric": METRIC,
-        "value": round(value, 1),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": round(host8 * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: U[-1,1) from the reference LCG (seed 1); lexicographic GLL ids",
-        "config": {"workload": f"BS8 gather+scatter FP64 (Add, Sym, K=1) on a {nx}^3 hex box at p={p} "
-                               f"({Nl} local / {Ng} global nodes); suite BS1-BS5 at N=2^{args.log2n}",
-                   "nx": nx, "degree": p, "local_nodes": Nl, "global_nodes": Ng, "vector_n": n,
-                   "bytes_per_step": int(bytes_bs8), "GDOFs": round(Ng / host8 / 1e9, 3),
-                   "pct_spec_peak": round(100 * value / SPEC_HBM_GBS, 1),
-                   "pct_measured_peak": round(100 * achieved / measured_peak, 1),
-                   "measured_peak_GBs": round(measured_peak, 1), "device": props["name"]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": SPEC_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / SPEC_HBM_GBS, 4), "traffic": traffic},
-        "cpu_baseline": cpu,
-        "suite": suite,
-    }
+    out = report(args, K, W, props, nx, p, Nl, Ng, n, bytes_bs8, host8, ev8, measured_peak, cpu, suite)
     print(json.dumps(out), flush=True)
 
 
