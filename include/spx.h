@@ -266,9 +266,12 @@ int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, v
    per-segment fold: x += alpha*p, r -= alpha*Ap (FP64 storage: in FP64;
    FP32 / BF16: in FP32, alpha rounded to FP32), rdotr[s] = sum over segment s
    of the stored r squared, in FP64.  No reference counterpart (the reference
-   runs one vector per launch, BS/BS5).  A plan fixes the split of segments
-   into workgroups (about target_blocks in all, in proportion to length); the
-   partials are folded in a fixed order, so results do not vary run to run. */
+   runs one vector per launch, BS/BS5).  A plan cuts every segment into tiles
+   at the multiples of 4096 entries and launches min(target_blocks,
+   tiles) workgroups that stride over the tiles; each tile leaves one FP64
+   partial and stage 2 folds each segment's partials in tile order, so results
+   depend neither on target_blocks nor on the run.  spx_cg_plan_blocks gives
+   the workgroups launched. */
 typedef struct spx_cg_plan_s *spx_cg_plan_t;
 int spx_cg_plan_create(int32_t nseg, const int64_t *segStarts, int32_t target_blocks, spx_cg_plan_t *plan);
 int spx_cg_plan_destroy(spx_cg_plan_t plan);

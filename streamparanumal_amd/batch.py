@@ -93,7 +93,7 @@ class Batch:
         """The same instances and union operator with values stored as another dtype."""
         return Batch(self.meshes, self.K, dtype, parts=self.parts, union=self.union)
 
-    def make_plan(self, target_blocks=1024):
+    def make_plan(self, target_blocks=2**31 - 1):
         self.plan = CgPlan(self.seg_starts, target_blocks)
 
     def step(self, alpha):

@@ -1,8 +1,13 @@
 // Batched CG update over many independent vectors ("segments") of uneven
 // length in one launch: x += alpha p, r -= alpha Ap on every segment, and one
-// FP64 rdotr per segment.  A host-built plan splits each segment into
-// contiguous block ranges in proportion to its length; stage 2 folds each
-// segment's block partials in block order, so every rdotr is deterministic.
+// FP64 rdotr per segment.  A host-built plan cuts each segment into tiles at
+// the multiples of 4096 entries; the workgroups of a fixed grid stride
+// over the tile list, so at any moment the grid works on one compact front
+// of each vector (contiguous per-workgroup ranges spread the concurrent
+// accesses over the whole batch, which on a 35 GB batch cost 15-20 % of
+// bandwidth in address translation).  Each tile leaves one FP64 partial;
+// stage 2 folds each segment's partials in tile order, so every rdotr is
+// deterministic and independent of the grid size and unroll.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,12 +24,18 @@ using namespace spx;
 
 struct spx_cg_plan_s {
   int32_t nseg = 0;
-  int32_t nblocks = 0;
+  int32_t ntiles = 0;
+  int32_t nblocks = 0;  // workgroups launched (grid-stride over the tiles)
   int64_t total = 0;
-  int64_t *blockRange = nullptr;  // device, 2*nblocks: [first, end) entries
-  int32_t *segBlocks = nullptr;   // device, nseg+1
-  double *partials = nullptr;     // device, nblocks
+  int64_t *tileRange = nullptr;  // device, 2*ntiles: [first, end) entries
+  int32_t *segTiles = nullptr;   // device, nseg+1
+  double *partials = nullptr;    // device, ntiles
 };
+
+namespace {
+constexpr int kTileChunks = 1024;                // chunks of 4 entries per tile: 4 per lane
+constexpr int64_t kTileEntries = 4 * kTileChunks;
+}  // namespace
 
 namespace {
 
@@ -99,59 +110,60 @@ template <bool NT, typename V> __device__ __forceinline__ void st(V *p, V v) {
   else *p = v;
 }
 
+// one tile's update; lane t takes chunks c0+t, c0+t+256, ... in order
 template <typename S, int U, bool NT>
-__global__ __launch_bounds__(256) void k_cg_batch1(const int64_t *__restrict__ blockRange, double alpha_d,
-                                                   const S *__restrict__ p_, const S *__restrict__ Ap_,
-                                                   S *__restrict__ x_, S *__restrict__ r_,
-                                                   double *__restrict__ partials) {
+__device__ __forceinline__ double cg_tile(int64_t first, int64_t end, typename Quad4<S>::C alpha,
+                                          const typename Quad4<S>::V *__restrict__ p,
+                                          const typename Quad4<S>::V *__restrict__ Ap,
+                                          typename Quad4<S>::V *__restrict__ x, typename Quad4<S>::V *__restrict__ r,
+                                          const S *__restrict__ p_, const S *__restrict__ Ap_, S *__restrict__ x_,
+                                          S *__restrict__ r_) {
   typedef Quad4<S> Q;
   typedef typename Q::V V;
   typedef typename Q::C C;
-  const C alpha = (C)alpha_d;
-  const V *__restrict__ p = reinterpret_cast<const V *>(p_);
-  const V *__restrict__ Ap = reinterpret_cast<const V *>(Ap_);
-  V *__restrict__ x = reinterpret_cast<V *>(x_);
-  V *__restrict__ r = reinterpret_cast<V *>(r_);
-  __shared__ double s[256];
   const int t = threadIdx.x;
-  const int64_t first = blockRange[2 * blockIdx.x], end = blockRange[2 * blockIdx.x + 1];
-  const int64_t cend = end / 4;  // whole chunks end here
-  int64_t c = first / 4 + t;
+  const int64_t c0 = first / 4, cend = end / 4;  // first is a multiple of 4
   double acc = 0.0;
-  for (; c + (int64_t)(U - 1) * 256 < cend; c += (int64_t)U * 256) {
-    V vp[U], vap[U], vx[U], vr[U];
+  if (cend - c0 == kTileChunks) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      vp[u] = ld<NT>(p + c + u * 256);
-      vap[u] = ld<NT>(Ap + c + u * 256);
-      vx[u] = ld<NT>(x + c + u * 256);
-      vr[u] = ld<NT>(r + c + u * 256);
+    for (int k = 0; k < kTileChunks / 256; k += U) {
+      V vp[U], vap[U], vx[U], vr[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t c = c0 + t + (k + u) * 256;
+        vp[u] = ld<NT>(p + c);
+        vap[u] = ld<NT>(Ap + c);
+        vx[u] = ld<NT>(x + c);
+        vr[u] = ld<NT>(r + c);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t c = c0 + t + (k + u) * 256;
+        C fp[4], fap[4], fx[4], fr[4];
+        Q::unpack(vp[u], fp);
+        Q::unpack(vap[u], fap);
+        Q::unpack(vx[u], fx);
+        Q::unpack(vr[u], fr);
+        update4(alpha, fp, fap, fx, fr);
+        st<NT>(x + c, Q::pack(fx));
+        st<NT>(r + c, Q::pack(fr));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc += (double)fr[e] * (double)fr[e];
+      }
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
+  } else {
+    for (int64_t c = c0 + t; c < cend; c += 256) {
       C fp[4], fap[4], fx[4], fr[4];
-      Q::unpack(vp[u], fp);
-      Q::unpack(vap[u], fap);
-      Q::unpack(vx[u], fx);
-      Q::unpack(vr[u], fr);
+      Q::unpack(p[c], fp);
+      Q::unpack(Ap[c], fap);
+      Q::unpack(x[c], fx);
+      Q::unpack(r[c], fr);
       update4(alpha, fp, fap, fx, fr);
-      st<NT>(x + c + u * 256, Q::pack(fx));
-      st<NT>(r + c + u * 256, Q::pack(fr));
+      x[c] = Q::pack(fx);
+      r[c] = Q::pack(fr);
 #pragma unroll
       for (int e = 0; e < 4; ++e) acc += (double)fr[e] * (double)fr[e];
     }
-  }
-  for (; c < cend; c += 256) {
-    C fp[4], fap[4], fx[4], fr[4];
-    Q::unpack(p[c], fp);
-    Q::unpack(Ap[c], fap);
-    Q::unpack(x[c], fx);
-    Q::unpack(r[c], fr);
-    update4(alpha, fp, fap, fx, fr);
-    x[c] = Q::pack(fx);
-    r[c] = Q::pack(fr);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) acc += (double)fr[e] * (double)fr[e];
   }
   // entries past the last whole chunk (end % 4 of them): lane 0, in order
   if (t == 0) {
@@ -179,22 +191,44 @@ __global__ __launch_bounds__(256) void k_cg_batch1(const int64_t *__restrict__ b
       acc += rs * rs;
     }
   }
-  s[t] = acc;
-  __syncthreads();
-#pragma unroll
-  for (int w = 128; w >= 1; w >>= 1) {
-    if (t < w) s[t] += s[t + w];
-    __syncthreads();
-  }
-  if (t == 0) partials[blockIdx.x] = s[0];
+  return acc;
 }
 
-// one workgroup per segment: its block partials in block order
-__global__ __launch_bounds__(256) void k_cg_batch2(const int32_t *__restrict__ segBlocks,
+// the workgroups of a fixed grid stride over the tiles; one partial per tile
+template <typename S, int U, bool NT>
+__global__ __launch_bounds__(256) void k_cg_batch1(const int64_t *__restrict__ tileRange, int32_t ntiles,
+                                                   double alpha_d, const S *__restrict__ p_,
+                                                   const S *__restrict__ Ap_, S *__restrict__ x_,
+                                                   S *__restrict__ r_, double *__restrict__ partials) {
+  typedef Quad4<S> Q;
+  typedef typename Q::V V;
+  typedef typename Q::C C;
+  const C alpha = (C)alpha_d;
+  const V *__restrict__ p = reinterpret_cast<const V *>(p_);
+  const V *__restrict__ Ap = reinterpret_cast<const V *>(Ap_);
+  V *__restrict__ x = reinterpret_cast<V *>(x_);
+  V *__restrict__ r = reinterpret_cast<V *>(r_);
+  __shared__ double s[4];
+  const int t = threadIdx.x;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t first = tileRange[2 * (int64_t)tile], end = tileRange[2 * (int64_t)tile + 1];
+    double v = cg_tile<S, U, NT>(first, end, alpha, p, Ap, x, r, p_, Ap_, x_, r_);
+    // fixed tree: butterfly inside each wave, then the four wave sums in order
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((t & 63) == 0) s[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) partials[tile] = (s[0] + s[1]) + (s[2] + s[3]);
+    __syncthreads();  // s is read before the next tile overwrites it
+  }
+}
+
+// one workgroup per segment: its tile partials in tile order
+__global__ __launch_bounds__(256) void k_cg_batch2(const int32_t *__restrict__ segTiles,
                                                    const double *__restrict__ partials, double *__restrict__ rdotr) {
   __shared__ double s[256];
   const int t = threadIdx.x;
-  const int b0 = segBlocks[blockIdx.x], b1 = segBlocks[blockIdx.x + 1];
+  const int b0 = segTiles[blockIdx.x], b1 = segTiles[blockIdx.x + 1];
   double a = 0.0;
   for (int i = b0 + t; i < b1; i += 256) a += partials[i];
   s[t] = a;
@@ -219,17 +253,17 @@ int launch_cg_batch(const spx_cg_plan_s &P, double alpha, const S *p, const S *A
   const dim3 g((unsigned)P.nblocks);
   hipStream_t st = current_stream();
 #define SPX_CGB(UU, NTT) \
-  hipLaunchKernelGGL((k_cg_batch1<S, UU, NTT>), g, dim3(256), 0, st, P.blockRange, alpha, p, Ap, x, r, P.partials)
+  hipLaunchKernelGGL((k_cg_batch1<S, UU, NTT>), g, dim3(256), 0, st, P.tileRange, P.ntiles, alpha, p, Ap, x, r, \
+                     P.partials)
   switch (tu.unroll) {
     case 1: if (nt) SPX_CGB(1, true); else SPX_CGB(1, false); break;
     case 2: if (nt) SPX_CGB(2, true); else SPX_CGB(2, false); break;
     case 4: if (nt) SPX_CGB(4, true); else SPX_CGB(4, false); break;
-    case 8: if (nt) SPX_CGB(8, true); else SPX_CGB(8, false); break;
-    default: return fail(SPX_ERR_ARG, "spx_cg_batch: unroll must be 1, 2, 4 or 8");
+    default: return fail(SPX_ERR_ARG, "spx_cg_batch: unroll must be 1, 2 or 4");
   }
 #undef SPX_CGB
   SPX_LAUNCHED("spx_cg_batch");
-  hipLaunchKernelGGL(k_cg_batch2, dim3((unsigned)P.nseg), dim3(256), 0, st, P.segBlocks, P.partials, rdotr);
+  hipLaunchKernelGGL(k_cg_batch2, dim3((unsigned)P.nseg), dim3(256), 0, st, P.segTiles, P.partials, rdotr);
   SPX_LAUNCHED("spx_cg_batch");
   return SPX_OK;
 }
@@ -251,38 +285,41 @@ int spx_cg_plan_create(int32_t nseg, const int64_t *segStarts, int32_t target_bl
   if (segStarts[0] < 0) return fail(SPX_ERR_ARG, "spx_cg_plan_create: negative start");
   const int64_t total = segStarts[nseg] - segStarts[0];
   std::vector<int64_t> range;
-  std::vector<int32_t> segBlocks(nseg + 1, 0);
+  std::vector<int32_t> segTiles(nseg + 1, 0);
   for (int32_t s = 0; s < nseg; ++s) {
-    const int64_t a = segStarts[s], len = segStarts[s + 1] - a;
-    int64_t nb = total > 0 ? (int64_t)std::llround((double)target_blocks * (double)len / (double)total) : 1;
-    const int64_t chunks = (len + 3) / 4;
-    nb = std::max<int64_t>(1, std::min<int64_t>(nb, std::max<int64_t>(1, chunks / 256)));
-    for (int64_t k = 0; k < nb; ++k) {
-      // block k: chunks [k*chunks/nb, (k+1)*chunks/nb) of the segment
-      const int64_t c0 = k * chunks / nb, c1 = (k + 1) * chunks / nb;
-      range.push_back(a + 4 * c0);
-      range.push_back(std::min(a + 4 * c1, a + len));
+    const int64_t a = segStarts[s], e = segStarts[s + 1];
+    // tiles end on multiples of kTileEntries from the vectors' start, so full tiles cover whole
+    // cache lines and neighbouring tiles share none; every segment has at least one tile (an
+    // empty one for an empty segment)
+    int64_t nt = 0;
+    for (int64_t k0 = a; k0 < e || nt == 0; ++nt) {
+      const int64_t k1 = std::min((k0 / kTileEntries + 1) * kTileEntries, e);
+      range.push_back(k0);
+      range.push_back(k1);
+      k0 = std::max(k1, k0 + 1);
     }
-    segBlocks[s + 1] = segBlocks[s] + (int32_t)nb;
+    if ((int64_t)segTiles[s] + nt > INT32_MAX) return fail(SPX_ERR_ARG, "spx_cg_plan_create: too many tiles");
+    segTiles[s + 1] = segTiles[s] + (int32_t)nt;
   }
   auto *P = new spx_cg_plan_s;
   P->nseg = nseg;
-  P->nblocks = segBlocks[nseg];
+  P->ntiles = segTiles[nseg];
+  P->nblocks = std::min(target_blocks, P->ntiles);
   P->total = total;
-  if (hipMalloc((void **)&P->blockRange, range.size() * sizeof(int64_t)) != hipSuccess ||
-      hipMalloc((void **)&P->segBlocks, segBlocks.size() * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc((void **)&P->partials, (size_t)P->nblocks * sizeof(double)) != hipSuccess) {
-    (void)hipFree(P->blockRange);
-    (void)hipFree(P->segBlocks);
+  if (hipMalloc((void **)&P->tileRange, range.size() * sizeof(int64_t)) != hipSuccess ||
+      hipMalloc((void **)&P->segTiles, segTiles.size() * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void **)&P->partials, (size_t)P->ntiles * sizeof(double)) != hipSuccess) {
+    (void)hipFree(P->tileRange);
+    (void)hipFree(P->segTiles);
     (void)hipFree(P->partials);
     delete P;
     return fail(SPX_ERR_HIP, "spx_cg_plan_create: device allocation failed");
   }
-  if (hipMemcpy(P->blockRange, range.data(), range.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(P->segBlocks, segBlocks.data(), segBlocks.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+  if (hipMemcpy(P->tileRange, range.data(), range.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(P->segTiles, segTiles.data(), segTiles.size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
           hipSuccess) {
-    (void)hipFree(P->blockRange);
-    (void)hipFree(P->segBlocks);
+    (void)hipFree(P->tileRange);
+    (void)hipFree(P->segTiles);
     (void)hipFree(P->partials);
     delete P;
     return fail(SPX_ERR_HIP, "spx_cg_plan_create: copy failed");
@@ -293,8 +330,8 @@ int spx_cg_plan_create(int32_t nseg, const int64_t *segStarts, int32_t target_bl
 
 int spx_cg_plan_destroy(spx_cg_plan_t plan) {
   if (!plan) return SPX_OK;
-  (void)hipFree(plan->blockRange);
-  (void)hipFree(plan->segBlocks);
+  (void)hipFree(plan->tileRange);
+  (void)hipFree(plan->segTiles);
   (void)hipFree(plan->partials);
   delete plan;
   return SPX_OK;

@@ -85,7 +85,7 @@ CgTuning AutotuneCgBatch(const std::vector<int64_t> &segStarts, int storage, voi
   spx_tuning_t keep;
   SPX_CHECK(spx_get_tuning("cg_batch", &keep));
   const int nseg = (int)segStarts.size() - 1;
-  for (int blocks : {1024, 2048, 4096, 8192}) {
+  for (int blocks : {2048, 8192, 32768, 131072, INT32_MAX}) {  // INT32_MAX: one workgroup per tile
     spx_cg_plan_t pl = nullptr;
     SPX_CHECK(spx_cg_plan_create(nseg, segStarts.data(), blocks, &pl));
     for (int unroll : {1, 2, 4})

@@ -50,10 +50,11 @@ def test_union_gather_scatter_per_instance(parts, oracle, st):
 @pytest.mark.parametrize("blocks", [1, 7, 64, 1024])
 def test_cg_batch_per_segment(dev, oracle, st, blocks):
     say("cg batch", st, blocks)
-    lens = [0, 4, 1024, 3, 1 << 16, 4099, 12]
+    lens = [0, 4, 1024, 3, 1 << 16, 4099, 12, 4096, 8196, 4101]
     starts = [0]
-    for n in lens:
+    for n in lens[:-1]:
         starts.append(starts[-1] + (n + 3) // 4 * 4)   # segments start on multiples of 4
+    starts.append(starts[-1] + lens[-1])               # the last one ends off a multiple of 4
     total = starts[-1]
     vecs64 = [oracle.uniform(total, s) for s in (11, 12, 13, 14)]
     vecs = [oracle.to_storage(v, st) if st != STORE_F64 else v for v in vecs64]
@@ -73,6 +74,34 @@ def test_cg_batch_per_segment(dev, oracle, st, blocks):
             ve, xe, re = oracle.cg_mixed(st, alpha, vecs[0][a:e], vecs[1][a:e], vecs[2][a:e], vecs[3][a:e])
         assert np.array_equal(xo[a:e], xe) and np.array_equal(ro[a:e], re)
         assert abs(rd[s] - ve) <= 1e-14 * abs(ve)
+
+
+@pytest.mark.parametrize("st", [STORE_F32, STORE_F64])
+def test_cg_batch_bits_do_not_depend_on_grid_or_unroll(dev, oracle, st):
+    """Every tile leaves one partial and each segment folds its partials in tile order, so x, r
+    and rdotr are the same bits for any grid size, unroll and store kind."""
+    say("cg batch grid independence", st)
+    from streamparanumal_amd._lib import libspx
+    lens = [40000, 4, 123456, 8192]
+    starts = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(starts[-1])
+    vecs64 = [oracle.uniform(total, s) for s in (51, 52, 53, 54)]
+    vecs = [oracle.to_storage(v, st) if st != STORE_F64 else v for v in vecs64]
+    results = []
+    for blocks in (1, 5, 64, 4096):
+        for unroll, nt in ((1, 0), (2, 1), (4, 0)):
+            device.set_tuning("cg_batch", unroll=unroll, nontemporal=nt)
+            try:
+                p, Ap, x, r = (DeviceArray.from_numpy(v) for v in vecs)
+                plan = CgPlan(starts, blocks)
+                plan.update(0.375, p, Ap, x, r)
+                device.check_kernel_errors()
+                results.append((x.to_numpy(), r.to_numpy(), plan.values()))
+            finally:
+                device.set_tuning("cg_batch")  # library defaults again
+    for xo, ro, rd in results[1:]:
+        assert np.array_equal(xo, results[0][0]) and np.array_equal(ro, results[0][1])
+        assert np.array_equal(rd.view(np.uint64), results[0][2].view(np.uint64))
 
 
 def test_cg_plan_rejects(dev):
@@ -123,7 +152,7 @@ def test_autotune_keeps_results(parts, oracle):
     b.step(0.5)
     for k, a in v.items():
         assert np.array_equal(a.to_numpy(), ref[k])
-    assert np.allclose(b.plan.values(), ref_rd, rtol=1e-14, atol=0)
+    assert np.array_equal(b.plan.values(), ref_rd)  # rdotr bits do not depend on the plan's grid
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
