@@ -75,17 +75,24 @@ def read_traffic():
 
 
 def cpu_baseline():
-    """The oracle's Serial restatement of BS8 on a 32^3 p=7 box, 1 host thread."""
+    """The oracle's Serial restatement of BS8 on a 32^3 p=7 box, 1 host thread (the JSON line's
+    cpu_baseline), and the same row loop under OpenMP on every host thread the process may use
+    (the reference's OpenMP mode; reported in the suite)."""
     from oracle import pyoracle
     pyoracle.lib()
     nx, p, reps = 32, 7, 30
-    r = pyoracle.time_bs8(nx, p, reps, 1)
-    nl, ng = r["nlocal"], r["nglobal"]
-    bytes_ = 4 * (ng + 1) + 4 * nl + 16 * nl
-    return {"value": round(bytes_ / r["seconds"] / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"BS8 gather+scatter FP64, {nx}^3 hex p={p} ({nl} local / {ng} global nodes), "
-                      f"{reps} calls of the oracle's Serial-mode restatement (gcc -O3), 1 thread; "
-                      f"{r['seconds'] * 1e3:.2f} ms/call, {ng / r['seconds'] / 1e9:.4f} GDOF/s"}
+    out = {}
+    for threads in (1, pyoracle.max_threads()):
+        r = pyoracle.time_bs8(nx, p, reps, threads)
+        nl, ng = r["nlocal"], r["nglobal"]
+        bytes_ = 4 * (ng + 1) + 4 * nl + 16 * nl
+        mode = "Serial-mode restatement" if threads == 1 else "OpenMP row loop"
+        out[threads] = {"value": round(bytes_ / r["seconds"] / 1e9, 3), "unit": "GB/s", "cores": threads,
+                        "kind": "port",
+                        "sample": f"BS8 gather+scatter FP64, {nx}^3 hex p={p} ({nl} local / {ng} global nodes), "
+                                  f"{reps} calls of the oracle's {mode} (gcc -O3), {threads} thread(s); "
+                                  f"{r['seconds'] * 1e3:.2f} ms/call, {ng / r['seconds'] / 1e9:.4f} GDOF/s"}
+    return out[1], out[max(out)]
 
 
 def batched_suite(K, W, count=64, seed=4, fields=3):
@@ -165,6 +172,78 @@ def batched_suite(K, W, count=64, seed=4, fields=3):
     log("batched", json.dumps(out))
     del b
     return out
+
+
+def precision_errors(alpha=0.31, seed=3):
+    """Errors of the FP32 / BF16 storage forms against this library's FP64 path (bit-exact with the
+    oracle) for the U[-1,1) and log-uniform |x| in [1e-6, 1] input sets of `seed`: BS8 on the 8^3
+    and 64^3 p=7 boxes, the BS5 CG update (x += alpha p, r -= alpha Ap, rdotr) at N = 262,144 and
+    2^27.  Relative errors are ||approx - exact|| / ||exact|| over the whole vector."""
+    import numpy as np
+
+    import streamparanumal_amd as spx
+    from streamparanumal_amd import device, ogs, streaming
+    from streamparanumal_amd.device import DeviceArray
+
+    fills = {"uniform": device.fill_uniform, "loguniform": device.fill_loguniform}
+    stores = ((spx.STORE_F32, np.float32, "f32"), (spx.STORE_BF16, np.uint16, "bf16"))
+    rows = []
+
+    def rel(red, approx, exact, diff):
+        red.norm(exact)
+        nref = red.value()
+        streaming.copy(approx, diff)
+        streaming.axpy(1.0, exact, -1.0, diff)
+        red.norm(diff)
+        return math.sqrt(red.value() / nref)
+
+    for nx, n5 in ((8, 262144), (64, 1 << 27)):
+        g = ogs.Ogs.hex(nx, nx, nx, 7, keep_ids=False)
+        Nl = g.N
+        red = streaming.Reducer(max(Nl, n5), spx.REDUCE_FAST, 256)
+        q, ref, back, diff = (DeviceArray(Nl) for _ in range(4))
+        for name, fill in fills.items():
+            fill(q, seed, 0)
+            streaming.copy(q, ref)
+            g.gather_scatter(ref)
+            for st, dt, tag in stores:
+                qs = DeviceArray(Nl, dt)
+                device.convert(q, qs)
+                g.gather_scatter_mixed(qs)
+                device.convert(qs, back)
+                rows.append({"kernel": "BS8", "size": f"{nx}^3 p=7", "inputs": name, "storage": tag,
+                             "rel_err": float(f"{rel(red, back, ref, diff):.3e}")})
+                del qs
+        del q, ref, back, diff, g
+        vec = [DeviceArray(n5) for _ in range(4)]          # p, Ap, x, r
+        x0, r0, back, diff = (DeviceArray(n5) for _ in range(4))
+        red_ref = streaming.Reducer(n5, spx.REDUCE_REFERENCE, 256)
+        mix = streaming.MixedCG()
+        for name, fill in fills.items():
+            for i, v in enumerate(vec):
+                fill(v, seed, i * n5)
+            streaming.copy(vec[2], x0)
+            streaming.copy(vec[3], r0)
+            red_ref.cg_full(alpha, vec[0], vec[1], x0, r0)
+            rd64 = red_ref.value()
+            for st, dt, tag in stores:
+                vs = [DeviceArray(n5, dt) for _ in range(4)]
+                for v, src in zip(vs, vec):
+                    device.convert(src, v)
+                mix.cg_full(alpha, *vs)
+                rd = mix.value()
+                device.convert(vs[2], back)
+                ex = rel(red, back, x0, diff)
+                device.convert(vs[3], back)
+                er = rel(red, back, r0, diff)
+                rows.append({"kernel": "BS5 CG update", "size": f"N={n5}", "inputs": name, "storage": tag,
+                             "rel_err_x": float(f"{ex:.3e}"), "rel_err_r": float(f"{er:.3e}"),
+                             "rel_err_rdotr": float(f"{abs(rd - rd64) / rd64:.3e}")})
+                del vs
+        del vec, x0, r0, back, diff, red, red_ref, mix
+    for r in rows:
+        log("precision", json.dumps(r))
+    return rows
 
 
 def degree_sweep(K, W, seed=2, target=1e8):
@@ -417,6 +496,9 @@ def main():
                            "graph_us_per_launch": round(replay * 1e6, 3)}
     log("BS0", json.dumps(suite["BS0_launch"]))
 
+    # ------------------------------------------- precision errors (config-1 and -2 sizes)
+    suite["precision_errors"] = precision_errors()
+
     # ------------------------------------------------------------ degree sweep
     if not args.no_degree_sweep:
         suite["degree_sweep_1e8"] = degree_sweep(K, W)
@@ -429,8 +511,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread) ...")
-        cpu = cpu_baseline()
-        log("cpu baseline", json.dumps(cpu))
+        cpu, cpu_all = cpu_baseline()
+        suite["cpu_baseline_all_threads"] = cpu_all
+        log("cpu baseline", json.dumps(cpu), json.dumps(cpu_all))
     out = report(args, K, W, props, nx, p, Nl, Ng, n, bytes_bs8, host8, ev8, measured_peak, cpu, suite)
     print(json.dumps(out), flush=True)
 

@@ -122,6 +122,11 @@ int spx_graph_destroy(spx_graph_t graph);
 /* the reference LCG (libs/primitives/random.cpp:45-163) mapped to U[-1,1):
    v[i] = -1 + (x_{skip+i} >> 11) * 2^-52 with x_0 = A*(UINT64_MAX ^ seed) + C */
 int spx_fill_uniform_f64(int64_t n, double *v, uint64_t seed, uint64_t skip);
+/* log-uniform magnitude in [1e-6, 1] with a random sign, from the same LCG:
+   v[i] = +-10^(-6u), u = (x_{2(skip+i)} >> 11) * 2^-53, negative when the top
+   bit of x_{2(skip+i)+1} is set (the device exp10 may differ from a host pow
+   in the last bit) */
+int spx_fill_loguniform_f64(int64_t n, double *v, uint64_t seed, uint64_t skip);
 /* element-wise conversion for the reduced-precision storage variants */
 int spx_convert_f64_to_f32(int64_t n, const double *src, float *dst);
 int spx_convert_f64_to_bf16(int64_t n, const double *src, uint16_t *dst);

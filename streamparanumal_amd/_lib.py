@@ -101,6 +101,7 @@ SPX_FUNCS = {
     "spx_graph_launch": (C.c_int, [vp]),
     "spx_graph_destroy": (C.c_int, [vp]),
     "spx_fill_uniform_f64": (C.c_int, [i64, vp, u64, u64]),
+    "spx_fill_loguniform_f64": (C.c_int, [i64, vp, u64, u64]),
     "spx_convert_f64_to_f32": (C.c_int, [i64, vp, vp]),
     "spx_convert_f64_to_bf16": (C.c_int, [i64, vp, vp]),
     "spx_convert_f32_to_f64": (C.c_int, [i64, vp, vp]),

@@ -101,6 +101,24 @@ __global__ void k_fill_uniform(int64_t n, double *v, uint64_t x0, uint64_t skip)
   }
 }
 
+// log-uniform magnitude in [1e-6, 1] with a random sign: element i takes draw
+// 2(skip+i) for the magnitude 10^(-6u), u = (x >> 11) * 2^-53, and the top bit
+// of draw 2(skip+i)+1 for the sign
+__global__ void k_fill_loguniform(int64_t n, double *v, uint64_t x0, uint64_t skip) {
+  const int64_t first = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kFillPerThread;
+  if (first >= n) return;
+  Affine j = aff_pow(Affine{kLcgA, kLcgC}, 2 * (skip + (uint64_t)first));
+  uint64_t x = j.a * x0 + j.c;
+  for (int e = 0; e < kFillPerThread; ++e) {
+    const int64_t i = first + e;
+    const double u = (double)(x >> 11) * 0x1p-53;
+    x = kLcgA * x + kLcgC;
+    const double mag = exp10(-6.0 * u);
+    if (i < n) v[i] = (x >> 63) ? -mag : mag;
+    x = kLcgA * x + kLcgC;
+  }
+}
+
 __global__ void k_f64_to_f32(int64_t n, const double *s, float *d) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
@@ -379,6 +397,21 @@ int spx_graph_destroy(spx_graph_t g) {
   SPX_HIP(hipGraphExecDestroy(g->exec));
   SPX_HIP(hipGraphDestroy(g->graph));
   delete g;
+  return SPX_OK;
+}
+
+int spx_fill_loguniform_f64(int64_t n, double *v, uint64_t seed, uint64_t skip) {
+  SPX_NEED_DEVICE();
+  if (n < 0) return fail(SPX_ERR_ARG, "spx_fill_loguniform_f64: negative n");
+  if (n == 0) return SPX_OK;
+  if (!v) return fail(SPX_ERR_ARG, "spx_fill_loguniform_f64: null v");
+  const uint64_t x0 = kLcgA * (UINT64_MAX ^ seed) + kLcgC;
+  const int threads = 256;
+  const int64_t nthreads = (n + kFillPerThread - 1) / kFillPerThread;
+  const int64_t blocks = (nthreads + threads - 1) / threads;
+  if (blocks > INT32_MAX) return fail(SPX_ERR_ARG, "spx_fill_loguniform_f64: n too large");
+  hipLaunchKernelGGL(k_fill_loguniform, dim3((unsigned)blocks), dim3(threads), 0, g_dev.current, n, v, x0, skip);
+  SPX_LAUNCHED("spx_fill_loguniform_f64");
   return SPX_OK;
 }
 

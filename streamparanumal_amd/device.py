@@ -167,6 +167,12 @@ def fill_uniform(arr, seed, skip=0):
     check(libspx().spx_fill_uniform_f64(arr.n, arr.ptr, seed, skip), "spx_fill_uniform_f64")
 
 
+def fill_loguniform(arr, seed, skip=0):
+    """+-10^(-6u), |x| log-uniform in [1e-6, 1], sign from the next draw's top bit: element i uses
+    draws 2(skip+i) and 2(skip+i)+1 of the reference LCG."""
+    check(libspx().spx_fill_loguniform_f64(arr.n, arr.ptr, seed, skip), "spx_fill_loguniform_f64")
+
+
 def convert(src, dst, n=None):
     """Convert between FP64 and a storage type on the device: float64 -> float32 / uint16 (bf16,
     nearest even), or float32 / uint16 -> float64 (exact)."""

@@ -33,6 +33,20 @@ def test_fill_uniform_matches_oracle(dev, oracle):
         assert np.array_equal(bits(d.to_numpy()), bits(oracle.uniform(n, seed, skip)))
 
 
+def test_fill_loguniform_matches_oracle(dev, oracle):
+    """Same draws as the oracle's host generator: identical signs, magnitudes in [1e-6, 1] within
+    2 ulp of the host pow (the device exp10 may round differently in the last bit)."""
+    for n, seed, skip in [(1, 3, 0), (1000, 3, 0), (4099, 0, 7), (100000, 3, 123456789)]:
+        say("fill loguniform", n, seed, skip)
+        d = DeviceArray(n)
+        device.fill_loguniform(d, seed, skip)
+        got, want = d.to_numpy(), oracle.loguniform(n, seed, skip)
+        assert np.array_equal(np.signbit(got), np.signbit(want))
+        assert np.all(np.abs(got) >= 1e-6 * (1 - 2 ** -50)) and np.all(np.abs(got) <= 1.0)
+        ulps = np.abs(bits(np.abs(got)).astype(np.int64) - bits(np.abs(want)).astype(np.int64))
+        assert ulps.max() <= 2
+
+
 @pytest.mark.parametrize("n", SIZES)
 def test_copy(dev, oracle, n):
     say("bs1 copy", n)
