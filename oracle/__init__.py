@@ -1,6 +1,6 @@
 """CPU oracle of the streaming hot path -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
-this package, and only as the checker (or the timed CPU baseline).  The
+this package, and only to check results against (or as the timed CPU baseline).  The
 product library never imports it.
 """

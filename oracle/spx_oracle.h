@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
  * Serial-mode semantics (OCCA "Serial" runs each OKL loop nest as sequential
- * host loops).  It is the checker for the HIP library: only tests/,
+ * host loops).  The HIP library's results are checked against it: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
  * product library never links, loads or calls it.
  *

@@ -225,7 +225,7 @@ void platform_t::DeviceConfig() {
   const std::string mode = settings().getSetting("THREAD MODEL");
   LIBP_ABORT("Thread model " << mode << " is not provided: this library runs its kernels as HIP on "
              "gfx950 (-m HIP).  The Serial/OpenMP restatement of the reference kernels is the test "
-             "oracle (oracle/), used only as a checker and CPU baseline.",
+             "oracle (oracle/), used only to check results and as the CPU baseline.",
              mode != "HIP");
   int dev = 0;
   settings().getSetting("DEVICE NUMBER", dev);

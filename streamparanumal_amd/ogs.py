@@ -249,6 +249,8 @@ class Ogs:
         checkh(libspxhost().spxh_ogs_set_scatter_path(self.h, 1 if by_rows else 0), "spxh_ogs_set_scatter_path")
 
     def global_to_local(self):
+        """SetupGlobalToLocalMapping (ogsSetup.cpp:833-849): the T-row of every local node, -1 where
+        the id is zero."""
         g = np.empty(self.N, dtype=np.int32)
         checkh(libspxhost().spxh_ogs_global_to_local(self.h, g.ctypes.data), "spxh_ogs_global_to_local")
         return g

@@ -394,3 +394,20 @@ def test_unflagged_scatter_leaves_zero_ids_alone(dev, oracle):
         assert same(dv.to_numpy(), want)
         assert same(dv.to_numpy()[::7], base[::7])
     g.set_scatter_path(True)
+
+
+@pytest.mark.parametrize("mesh", [(2, 2, 1, 2), (3, 2, 2, 3)])
+def test_global_to_local_mapping(dev, oracle, mesh):
+    """SetupGlobalToLocalMapping (ogsSetup.cpp:833-849): the T-row of every local node, -1 for nodes
+    whose id is zero (left out of the operator)."""
+    say("global to local", mesh)
+    ids = ogs.hex_ids(*mesh).copy()
+    ids[::7] = 0
+    g = ogs.Ogs(ids)
+    o = oracle.Ogs(ids)
+    want = np.full(ids.size, -1, dtype=np.int32)
+    for r in range(o.NrowsT):
+        want[o.colIdsT[o.rowStartsT[r]:o.rowStartsT[r + 1]]] = r
+    got = g.global_to_local()
+    assert np.array_equal(got, want)
+    assert np.all(got[::7] == -1)
