@@ -80,4 +80,39 @@ inline void fillUniform(deviceMemory<dfloat> &m, unsigned long long seed, unsign
   if (m.length()) SPX_CHECK(spx_fill_uniform_f64((int64_t)m.length(), m.ptr(), seed, skip));
 }
 
+// --storage: the stored value type of the reduced-precision forms (sums and
+// reductions stay FP64); FP64 runs the plain kernels
+inline void addStorage(settings_t &s) {
+  s.newSetting("--storage", "--storage", "STORAGE", "FP64",
+               "Stored value type (FP32 / BF16: FP64 row sums and reductions)", {"FP64", "FP32", "BF16"});
+}
+inline int storageOf(settings_t &s) {
+  if (s.compareSetting("STORAGE", "FP32")) return SPX_STORE_F32;
+  if (s.compareSetting("STORAGE", "BF16")) return SPX_STORE_BF16;
+  return SPX_STORE_F64;
+}
+inline size_t storageBytes(int storage) {
+  return storage == SPX_STORE_F64 ? 8 : storage == SPX_STORE_F32 ? 4 : 2;
+}
+
+// n values of U[-1,1) from the LCG (draws skip..skip+n-1), rounded to the
+// storage type (nearest even) into a device array of n * storageBytes bytes
+inline deviceMemory<unsigned char> stored(platform_t &platform, int storage, size_t n, unsigned long long seed,
+                                          unsigned long long skip) {
+  deviceMemory<unsigned char> out = platform.malloc<unsigned char>(n * storageBytes(storage));
+  if (!n) return out;
+  if (storage == SPX_STORE_F64) {
+    SPX_CHECK(spx_fill_uniform_f64((int64_t)n, reinterpret_cast<double *>(out.ptr()), seed, skip));
+    return out;
+  }
+  deviceMemory<dfloat> tmp = platform.malloc<dfloat>(n);
+  SPX_CHECK(spx_fill_uniform_f64((int64_t)n, tmp.ptr(), seed, skip));
+  if (storage == SPX_STORE_F32)
+    SPX_CHECK(spx_convert_f64_to_f32((int64_t)n, tmp.ptr(), reinterpret_cast<float *>(out.ptr())));
+  else
+    SPX_CHECK(spx_convert_f64_to_bf16((int64_t)n, tmp.ptr(), reinterpret_cast<uint16_t *>(out.ptr())));
+  platform.finish();
+  return out;
+}
+
 }  // namespace bs
