@@ -55,6 +55,20 @@ def test_rand_int_is_abs_of_low_word():
         assert g == abs(s)
 
 
+def test_loguniform_draw_order():
+    """Element i: magnitude 10^(-6u) from draw 2(skip+i), u = (x >> 11) 2^-53, sign from the top bit
+    of draw 2(skip+i)+1 -- recomputed from the Python LCG with the same libm pow."""
+    import math
+    n, seed, skip = 500, 3, 17
+    xs = lcg_python(seed, 2 * (skip + n))
+    want = np.empty(n)
+    for i in range(n):
+        u = (xs[2 * (skip + i)] >> 11) * 2.0 ** -53
+        mag = math.pow(10.0, -6.0 * u)
+        want[i] = -mag if xs[2 * (skip + i) + 1] >> 63 else mag
+    assert np.array_equal(o.loguniform(n, seed, skip).view(np.uint64), want.view(np.uint64))
+
+
 def test_loguniform_range_and_signs():
     v = o.loguniform(20000, 3)
     a = np.abs(v)
