@@ -246,6 +246,49 @@ def precision_errors(alpha=0.31, seed=3):
     return rows
 
 
+def element_types(K, W, seed=2):
+    """FP64 gather (Trans), scatter (NoTrans) and gather+scatter (Sym) on a box of each element type
+    at p=3 and p=7, sized to about 1e8 local nodes, set up through the general ogs setup from the
+    box ids (the hex box through its closed form)."""
+    import streamparanumal_amd as spx
+    from streamparanumal_amd import device, ogs
+    from streamparanumal_amd.device import DeviceArray
+
+    rows = []
+    for el in ("Tri", "Quad", "Tet", "Hex"):
+        for p in (3, 7):
+            _, npe = ogs.box_shape(el, 1, 1, 1, p)
+            per_cell = {"Tri": 2, "Quad": 1, "Tet": 6, "Hex": 1}[el] * npe
+            dim = 2 if el in ("Tri", "Quad") else 3
+            n = int(round((1e8 / per_cell) ** (1.0 / dim)))
+            nz = n if dim == 3 else 1
+            t0 = time.perf_counter()
+            if el == "Hex":
+                g = ogs.Ogs.hex(n, n, n, p, keep_ids=False)
+            else:
+                g = ogs.Ogs(ogs.box_ids(el, n, n, nz, p))
+            setup = time.perf_counter() - t0
+            Nl, Ng = g.N, g.NgatherGlobal
+            q = DeviceArray(Nl)
+            device.fill_uniform(q, seed, 0)
+            gq = DeviceArray(Ng)
+            b6 = 4 * (Ng + 1) + 12 * Nl + 8 * Ng
+            b7 = 4 * (Ng + 1) + 4 * Nl + 8 * Ng + 8 * Nl
+            b8 = 4 * (Ng + 1) + 4 * Nl + 16 * Nl
+            _, e6 = timed(lambda: g.gather(gq, q, 1, spx.ADD, spx.TRANS), K, W)
+            _, e7 = timed(lambda: g.scatter(q, gq, 1, spx.NOTRANS), K, W)
+            _, e8 = timed(lambda: g.gather_scatter(q), K, W)
+            device.check_kernel_errors()
+            row = {"element": el, "p": p, "n": n, "local_nodes": Nl, "global_nodes": Ng,
+                   "Ng_over_Nl": round(Ng / Nl, 3), "setup_s": round(setup, 2),
+                   "gather_GBs": round(b6 / e6 / 1e9, 1), "scatter_GBs": round(b7 / e7 / 1e9, 1),
+                   "gather_scatter_GBs": round(b8 / e8 / 1e9, 1), "gather_scatter_GDOFs": round(Ng / e8 / 1e9, 2)}
+            rows.append(row)
+            log("element types", json.dumps(row))
+            del g, q, gq
+    return rows
+
+
 def degree_sweep(K, W, seed=2, target=1e8):
     """FP64 gather (Trans), scatter (NoTrans) and gather+scatter (Sym) on hex boxes of
     round((target/(p+1)^3)^(1/3)) elements per side for p = 1..15 (about `target` local nodes
@@ -502,6 +545,7 @@ def main():
     # ------------------------------------------------------------ degree sweep
     if not args.no_degree_sweep:
         suite["degree_sweep_1e8"] = degree_sweep(K, W)
+        suite["element_types_1e8"] = element_types(K, W)
 
     # ------------------------------------------------------- batched instances
     if not args.no_batched:

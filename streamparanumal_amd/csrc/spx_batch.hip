@@ -210,8 +210,8 @@ __global__ __launch_bounds__(256) void k_cg_batch1(const int64_t *__restrict__ t
   V *__restrict__ r = reinterpret_cast<V *>(r_);
   __shared__ double s[4];
   const int t = threadIdx.x;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t first = tileRange[2 * (int64_t)tile], end = tileRange[2 * (int64_t)tile + 1];
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t first = tileRange[2 * tile], end = tileRange[2 * tile + 1];
     double v = cg_tile<S, U, NT>(first, end, alpha, p, Ap, x, r, p_, Ap_, x_, r_);
     // fixed tree: butterfly inside each wave, then the four wave sums in order
 #pragma unroll
