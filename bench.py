@@ -367,6 +367,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-mesh batched comparison")
     ap.add_argument("--no-degree-sweep", action="store_true", help="skip the p=1..15 sweep at ~1e8 nodes")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run the 64-mesh batched comparison alone and print its JSON: the command the committed "
+                         "rocprofv3 summary of the batched kernels is taken from")
     ap.add_argument("--only-bs8", action="store_true",
                     help="time the BS8 step alone (no suite, no CPU baseline): the command the committed "
                          "rocprofv3 summaries of the BS8 kernel are taken from")
@@ -386,6 +389,9 @@ def main():
     ogs.platform_open(0)
     props = device.props()
     log("device", props["name"], props["arch"], props["compute_units"], "CUs")
+    if args.only_batched:
+        print(json.dumps({"batched_64_meshes": batched_suite(K, W)}), flush=True)
+        return
 
     # ---------------------------------------------------------------- BS6-BS8
     nx, p = args.nx, args.degree
