@@ -73,3 +73,25 @@ def test_bs0_plain_and_graph():
 def test_unknown_flag_is_refused():
     r = run("BS1", "--no-such-flag", "1")
     assert r.returncode != 0
+
+
+def sweep_rows(out, name):
+    """Numeric rows of a sweep's MATLAB block `name = [ ... ];`."""
+    text = out.split(name + " = [", 1)[1].split("]", 1)[0]
+    return [ln.strip().rstrip(";").split() for ln in text.strip().splitlines() if ln.strip()]
+
+
+def test_byte_sweep():
+    mb = 1 << 20
+    r = run("BS1", "-bmin", str(16 * mb), "-bmax", str(64 * mb), "-s", str(16 * mb))
+    assert r.returncode == 0, r.stderr[-2000:]
+    rows = sweep_rows(r.stdout, "BS1")
+    assert [int(x[0]) for x in rows] == [mb, 2 * mb, 3 * mb, 4 * mb]   # N = bytes / 16
+    assert all(float(x[3]) > 0 for x in rows)
+
+
+def test_launch_sweep():
+    r = run("BS0", "-nmin", "100", "-nmax", "400", "-s", "100")
+    assert r.returncode == 0, r.stderr[-2000:]
+    rows = sweep_rows(r.stdout, "BS0")
+    assert [int(x[0]) for x in rows] == [100, 200, 300, 400]
