@@ -26,4 +26,9 @@ class comm_t {
   template <typename T> void Scan(const T &snd, T &rcv, const Comm::op_t = Comm::Sum) const { rcv = snd; }
 };
 
+namespace Comm {
+// the world communicator (include/comm.hpp:69-92)
+inline comm_t World() { return comm_t::world(); }
+}  // namespace Comm
+
 }  // namespace libp

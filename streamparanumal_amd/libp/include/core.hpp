@@ -29,6 +29,10 @@ class exception : public std::exception {
 
 }  // namespace libp
 
+// program exit codes (include/utils.hpp:50-51)
+#define LIBP_SUCCESS 0
+#define LIBP_ERROR -1
+
 #define LIBP_FORCE_ABORT(msg)                                                  \
   do {                                                                         \
     std::stringstream libp_ss_;                                                \
