@@ -47,14 +47,21 @@ spx_tuning_t tuning_for(const char *kernel, spx_tuning_t defaults);
 
 // ---- device helpers ----------------------------------------------------------------
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
-// with a private L2.  Remap the flat workgroup id so that XCD x runs one
-// contiguous range of tiles: neighbouring tiles (which share cache lines at
-// their edges) then meet in the same L2.  Bijective on [0, n).
+// with a private L2.  Remap the flat workgroup id so that XCD x takes blocks
+// of kXcdBlock consecutive tiles in turn (tiles g*nxcd*B + x*B ... + B - 1 of
+// group g): neighbouring tiles, which share cache lines at their edges, meet
+// in one L2, while the XCDs stay within nxcd*B tiles of each other.  (One
+// contiguous range per XCD put the XCDs' streams a power of two apart in
+// memory on power-of-two meshes, 2 % slower there; blocks of 128 tiles were
+// the fastest of 16..1024 on 64^3 p=7, 58^3 p=7, 116^3 p=3 and 46^3 p=9.)
+// Tiles past the last whole group keep their id.  Bijective on [0, n).
+constexpr int kXcdBlock = 128;
 __device__ __forceinline__ int xcd_tile(int bid, int n, int nxcd) {
   if (nxcd <= 1) return bid;
-  const int q = n / nxcd, r = n % nxcd;
-  const int x = bid % nxcd, l = bid / nxcd;
-  return (x < r) ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+  const int group = nxcd * kXcdBlock;
+  const int l = bid / nxcd, x = bid % nxcd;
+  if (l >= (n / group) * kXcdBlock) return bid;
+  return (l / kXcdBlock) * group + x * kXcdBlock + (l % kXcdBlock);
 }
 
 inline bool aligned(const void *p, size_t a) {
