@@ -419,9 +419,12 @@ static void launch_ref1(int64_t n, int64_t nb, const RedArgs &A, double *tmp, in
 // Measured on MI355X at N = 2^27 (TB/s, unroll 16): streaming loads/stores
 // give norm 6.81 / dot 6.93 / update 5.97 / full form 5.78, against 6.20 /
 // 6.36 / 4.89 / 5.17 with plain ones.
-static spx_tuning_t reference_tuning() {
+// Unroll 16 for norm, dot and the full form; 8 for the update (6.07-6.10
+// against 5.99 TB/s at unroll 16; norm 6.97, dot 6.95-7.00, full form 5.84
+// at 16, each within 1 % or better than 8 and 32 there).
+static spx_tuning_t reference_tuning(int kind) {
   spx_tuning_t d{};
-  d.unroll = 16;
+  d.unroll = kind == RED_CG ? 8 : 16;
   d.nontemporal = 1;
   d.xcd_remap = 0;
   d.variant = 0;
@@ -439,7 +442,7 @@ static int run_reduction(const char *name, int64_t n, const RedArgs &A, double *
       return fail(SPX_ERR_ARG, std::string(name) + ": block_size must be 256, 512 or 1024");
     int64_t nb = (n + bs - 1) / bs;
     if (nb > bs) nb = bs;
-    const spx_tuning_t t = reference_tuning();
+    const spx_tuning_t t = reference_tuning(KIND);
     const int U = t.unroll > 0 ? t.unroll : 16;
     const bool nt = t.nontemporal > 0;
     if (nb > 0) {
@@ -484,7 +487,7 @@ static int run_stage1(const char *name, int64_t nblocks, int64_t n, const RedArg
   if (nblocks > INT32_MAX) return fail(SPX_ERR_ARG, std::string(name) + ": too many blocks");
   if (nblocks == 0) return SPX_OK;
   if (!partials) return fail(SPX_ERR_ARG, std::string(name) + ": null partials");
-  const spx_tuning_t t = reference_tuning();
+  const spx_tuning_t t = reference_tuning(KIND);
   const int U = t.unroll > 0 ? t.unroll : 16;
   const bool nt = t.nontemporal > 0;
   if (bs == 256) launch_ref1<KIND, 256>(n, nblocks, A, partials, U, nt);
