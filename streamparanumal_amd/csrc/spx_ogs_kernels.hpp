@@ -15,8 +15,9 @@
 //   * phase 3: lanes write results back in CSR order;
 //   * no atomics: every local node belongs to exactly one row and every row to
 //     exactly one block, so the in-place update is race-free;
-//   * workgroup ids are remapped so that XCD x owns one contiguous range of
-//     row blocks (neighbouring blocks share cache lines at their edges).
+//   * workgroup ids are remapped so that each XCD takes blocks of consecutive
+//     row blocks in turn (neighbouring blocks share cache lines at their
+//     edges; xcd_tile).
 // K fields per node (AoS, stride K) are processed together (KF = K <= 4) or
 // one field per grid row (KF = 1, grid.y = K), like the reference's
 // @outer(1) loop.
