@@ -113,15 +113,20 @@ int launch_gs_srb_mixed(const Launch &L, const SrbArgs &A, S *q) {
   Launch W = L;
   const bool wave = L.variant == 2 && L.cap <= 16 * 64;
   const int lanes = wave ? 64 : kThreads;
-  W.per = pick_per(W.kf, W.cap * (kThreads / lanes));
-  if (!W.per) {
-    W.kf = 1;
-    W.per = pick_per(1, W.cap * (kThreads / lanes));
+  // all fields per pass if they fit the registers and LDS, else half (K even), else one
+  auto lds_of = [&](int kf) { return (size_t)W.cap * kf * sizeof(S) + (size_t)W.cap * sizeof(unsigned short); };
+  auto fits = [&](int kf) {
+    W.kf = kf;
+    W.per = pick_per(kf, W.cap * (kThreads / lanes));
+    return W.per > 0 && lds_of(kf) <= 65536;
+  };
+  const int kf0 = L.kf;
+  if (!fits(kf0) && !(kf0 > 2 && kf0 % 2 == 0 && fits(kf0 / 2)) && !fits(1)) {
+    if (!W.per) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile too large");
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile too large for LDS");
   }
-  if (!W.per) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile too large");
-  const size_t lds = (size_t)W.cap * W.kf * sizeof(S) + (size_t)W.cap * sizeof(unsigned short);
-  if (lds > 65536) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: tile too large for LDS");
-  const dim3 grid(W.nblocks, W.kf == 1 ? W.K : 1);
+  const size_t lds = lds_of(W.kf);
+  const dim3 grid(W.nblocks, W.K / W.kf);
 #define SPX_GSRM(KF, PER)                                                                                   \
   do {                                                                                                     \
     if (wave)                                                                                              \

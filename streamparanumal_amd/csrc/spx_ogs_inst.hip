@@ -37,7 +37,8 @@ static int dispatch(int op, int kf, int per, F &&f) {
   }
 }
 
-static dim3 grid_of(const Launch &L) { return dim3(L.nblocks, L.kf == 1 ? L.K : 1); }
+// one grid row per pass of kf fields (kf divides K)
+static dim3 grid_of(const Launch &L) { return dim3(L.nblocks, L.K / L.kf); }
 
 int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const int *ids, T *q) {
   const size_t rsb = L.variant == 1 ? (size_t)(L.cap + 1) * sizeof(int)
@@ -58,21 +59,27 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
 }
 
 // sorted-run kernels: one-wave tiles (variant 2, up to 1024 positions) or 256
-// lanes; fields one per pass when a multi-field tile does not fit the lanes'
-// registers.  L.cap is the layout's positions per tile (chunks per tile * 64).
+// lanes.  All K fields go in one pass when the tile's registers and LDS hold
+// them; otherwise half of them per pass (K even), else one per pass (e.g.
+// FP64, K = 4 on 2048-position tiles: 64 KB of values, two passes of two
+// fields).  L.cap is the layout's positions per tile (chunks per tile * 64).
 template <bool WITH_OP, typename F>
 static int srb_shape(const Launch &L, int op, F &&f) {
   Launch W = L;
   const bool wave = L.variant == 2 && L.cap <= 16 * 64;
   const int lanes = wave ? 64 : kThreads;
-  W.per = pick_per(W.kf, W.cap * (kThreads / lanes));
-  if (!W.per) {
-    W.kf = 1;
-    W.per = pick_per(1, W.cap * (kThreads / lanes));
+  auto lds_of = [&](int kf) { return lds_bytes(W.cap, kf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short); };
+  auto fits = [&](int kf) {
+    W.kf = kf;
+    W.per = pick_per(kf, W.cap * (kThreads / lanes));
+    return W.per > 0 && lds_of(kf) <= 65536;
+  };
+  const int kf0 = L.kf;
+  if (!fits(kf0) && !(kf0 > 2 && kf0 % 2 == 0 && fits(kf0 / 2)) && !fits(1)) {
+    if (!W.per) return fail(SPX_ERR_ARG, "ogs sorted-run: tile too large");
+    return fail(SPX_ERR_ARG, "ogs sorted-run: tile too large for LDS");
   }
-  if (!W.per) return fail(SPX_ERR_ARG, "ogs sorted-run: tile too large");
-  const size_t lds = lds_bytes(W.cap, W.kf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short);
-  if (lds > 65536) return fail(SPX_ERR_ARG, "ogs sorted-run: tile too large for LDS");
+  const size_t lds = lds_of(W.kf);
   return dispatch<WITH_OP>(op, W.kf, W.per, [&]<int OPC, int KF, int PER>() {
     return f.template operator()<OPC, KF, PER>(W, wave, lds);
   });

@@ -109,7 +109,7 @@ def test_bs6_gather_bs7_scatter_double(setups, oracle, mesh):
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("op", OPS)
 @pytest.mark.parametrize("trans", TRANS)
-@pytest.mark.parametrize("k", [1, 2, 3, 5])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
 def test_all_types_ops_trans_fields(setups, dtype, op, trans, k):
     mesh = (3, 2, 1, 3)
     say("gs/g/s", mesh, np.dtype(dtype).name, op, trans, k)
@@ -411,3 +411,18 @@ def test_global_to_local_mapping(dev, oracle, mesh):
     got = g.global_to_local()
     assert np.array_equal(got, want)
     assert np.all(got[::7] == -1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.int64])
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_multi_field_gather_scatter_full_tiles(setups, dtype, k):
+    """K fields in one pass (word-major access over whole 64-position chunks) on a mesh of many full
+    tiles: bit-exact with the oracle."""
+    mesh = (8, 8, 8, 7)
+    say("multi-field gs", mesh, np.dtype(dtype).name, k)
+    g, o, _ = setups[mesh]
+    q = values(g.N, dtype, 12, k)
+    dq = DeviceArray.from_numpy(q)
+    g.gather_scatter(dq, k)
+    device.check_kernel_errors()
+    assert same(dq.to_numpy(), o.gather_scatter(q, k))
