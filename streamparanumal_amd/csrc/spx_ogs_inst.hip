@@ -63,12 +63,17 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
 // them; otherwise half of them per pass (K even), else one per pass (e.g.
 // FP64, K = 4 on 2048-position tiles: 64 KB of values, two passes of two
 // fields).  L.cap is the layout's positions per tile (chunks per tile * 64).
+// gs: the gather+scatter kernel, which keeps srb_lds_fields<T>(kf) fields of
+// a position in LDS (the gather and scatter kernels keep all kf).
 template <bool WITH_OP, typename F>
-static int srb_shape(const Launch &L, int op, F &&f) {
+static int srb_shape(const Launch &L, int op, F &&f, bool gs = false) {
   Launch W = L;
   const bool wave = L.variant == 2 && L.cap <= 16 * 64;
   const int lanes = wave ? 64 : kThreads;
-  auto lds_of = [&](int kf) { return lds_bytes(W.cap, kf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short); };
+  auto lds_of = [&](int kf) {
+    const int lf = gs ? srb_lds_fields<T>(kf) : kf;
+    return lds_bytes(W.cap, lf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short);
+  };
   auto fits = [&](int kf) {
     W.kf = kf;
     W.per = pick_per(kf, W.cap * (kThreads / lanes));
@@ -86,13 +91,18 @@ static int srb_shape(const Launch &L, int op, F &&f) {
 }
 
 int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q) {
-  return srb_shape<true>(L, op, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
-    if (wave)
-      hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A), q);
-    else
-      hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), q);
-    return SPX_OK;
-  });
+  return srb_shape<true>(
+      L, op,
+      [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
+        if (wave)
+          hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A),
+                             q);
+        else
+          hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W,
+                             SRB_ARGS(A), q);
+        return SPX_OK;
+      },
+      true);
 }
 
 int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const T *v, T *gv) {

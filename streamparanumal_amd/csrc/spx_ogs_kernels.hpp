@@ -302,11 +302,30 @@ __device__ __forceinline__ void srb_decode(const SrbChunk *__restrict__ chunks, 
   }
 }
 
+// Values a sorted-run gather+scatter tile keeps in LDS per position: every
+// field for 2-byte values (word-major, below), one field at a time otherwise.
+template <typename T> constexpr int srb_lds_fields(int kf) { return (kf > 1 && sizeof(T) == 2) ? kf : 1; }
+
 template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>>
 __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restrict__ q) {
+  // Several 4- or 8-byte fields per pass: each lane loads and stores its
+  // position's KF values as one wide access (the compiler merges them) and
+  // keeps them in registers, and the fields pass through LDS one at a time,
+  // so a tile needs the LDS of one field whatever KF (FP64, K = 3 on
+  // 2048-position tiles: 20 KB instead of 52 KB).
+  //
+  // Several 2-byte fields per pass go word-major instead: the wave's chunk j
+  // holds 64*KF values (KF fields of 64 positions, contiguous per position);
+  // access f takes words 64f .. 64f+63 of them, lane l the word w = 64f + l =
+  // field w % KF of position w / KF, whose local index comes from that
+  // position's lane.  One access then spans a run's consecutive words instead
+  // of KF separate 2-byte accesses per lane (BF16, K = 3: 13 % faster), and
+  // the LDS slot of word w is the chunk's base slot + w.
+  constexpr bool kWord = KF > 1 && sizeof(T) == 2;
+  constexpr int LF = kWord ? KF : 1;  // fields held in LDS per position
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
-  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
+  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * LF);
   const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
   const int kb = blockIdx.y * KF;
   const int cap = L.cap;  // cpt * 64 positions
@@ -319,16 +338,6 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
   unsigned valid;
   srb_decode<PER, NT>(chunks, runIds, links, b, cap / 64, id, prev, row, valid);
   (void)row;
-  // Word-major access for several 2-byte fields per pass.  The wave's chunk j
-  // holds 64*KF values (KF fields of 64 positions, contiguous per position);
-  // access f takes words 64f .. 64f+63 of them, lane l the word w = 64f + l =
-  // field w % KF of position w / KF, whose local index comes from that
-  // position's lane.  One access then spans a run's consecutive words instead
-  // of KF separate 2-byte accesses per lane, and the LDS slot of word w is the
-  // chunk's base slot + w.  (Wider values need no such help: the compiler
-  // merges a position's fields into one wide access; BF16 with K = 3 runs
-  // 13 % faster word-major, FP32 and FP64 5-10 % slower.)
-  constexpr bool kWord = KF > 1 && sizeof(T) == 2;
   const int l = t & 63;
   const int cb0 = t - l;  // first position of this wave's chunk 0
   T v[PER][KF];
@@ -344,44 +353,57 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
         v[j][f] = q[(int64_t)idw * K + kb + (w - src * KF)];
       }
     }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    if constexpr (!kWord) {
-      if ((valid >> j) & 1u)
-#pragma unroll
-        for (int f = 0; f < KF; ++f) vals[(t + j * NT) * KF + f] = v[j][f];
-    } else {
-      const unsigned long long vm = __ballot((valid >> j) & 1u);
-#pragma unroll
-      for (int f = 0; f < KF; ++f) {
-        const int w = f * 64 + l;
-        if ((vm >> (w / KF)) & 1ull) vals[(cb0 + j * NT) * KF + w] = v[j][f];
-      }
-    }
-    if (((valid >> j) & 1u) && prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
-  }
-  __syncthreads();
+  __syncthreads();  // next[] initialised before the links go in
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (((valid >> j) & 1u) && prev[j] < 0) {
-      const int h = t + j * NT;
+    if (((valid >> j) & 1u) && prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
+  // one LDS round per field (one round in all for KF == 1 and word-major)
+  constexpr int ROUNDS = kWord ? 1 : KF;
 #pragma unroll
-      for (int f = 0; f < KF; ++f) {
-        typename F::Acc acc = F::init();
-        // walks end at kEnd (> cap); the step bound also ends them on a malformed layout
-        for (int i = h, s = 0; i < cap && s < cap; i = next[i], ++s) F::apply(acc, vals[i * KF + f]);
-        const T r = F::out(acc);
-        for (int i = h, s = 0; i < cap && s < cap; i = next[i], ++s) vals[i * KF + f] = r;
+  for (int f0 = 0; f0 < ROUNDS; ++f0) {
+    if (f0 > 0) __syncthreads();  // the previous field has been read back
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if constexpr (!kWord) {
+        if ((valid >> j) & 1u) vals[t + j * NT] = v[j][f0];
+      } else {
+        const unsigned long long vm = __ballot((valid >> j) & 1u);
+#pragma unroll
+        for (int f = 0; f < KF; ++f) {
+          const int w = f * 64 + l;
+          if ((vm >> (w / KF)) & 1ull) vals[(cb0 + j * NT) * KF + w] = v[j][f];
+        }
       }
     }
-  __syncthreads();
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (((valid >> j) & 1u) && prev[j] < 0) {
+        const int h = t + j * NT;
+#pragma unroll
+        for (int f = 0; f < LF; ++f) {
+          typename F::Acc acc = F::init();
+          // walks end at kEnd (> cap); the step bound also ends them on a malformed layout
+          for (int i = h, s = 0; i < cap && s < cap; i = next[i], ++s) F::apply(acc, vals[i * LF + f]);
+          const T r = F::out(acc);
+          for (int i = h, s = 0; i < cap && s < cap; i = next[i], ++s) vals[i * LF + f] = r;
+        }
+      }
+    __syncthreads();
+    if constexpr (!kWord && KF > 1) {
+#pragma unroll
+      for (int j = 0; j < PER; ++j)
+        if ((valid >> j) & 1u) v[j][f0] = vals[t + j * NT];
+    }
+  }
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
-    if constexpr (!kWord) {
+    if constexpr (KF == 1) {
+      if ((valid >> j) & 1u) q[(int64_t)id[j] * K + kb] = vals[t + j * NT];
+    } else if constexpr (!kWord) {
       if ((valid >> j) & 1u)
 #pragma unroll
-        for (int f = 0; f < KF; ++f) q[(int64_t)id[j] * K + kb + f] = vals[(t + j * NT) * KF + f];
+        for (int f = 0; f < KF; ++f) q[(int64_t)id[j] * K + kb + f] = v[j][f];
     } else {
       const unsigned long long vm = __ballot((valid >> j) & 1u);
 #pragma unroll
