@@ -64,14 +64,14 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
 // FP64, K = 4 on 2048-position tiles: 64 KB of values, two passes of two
 // fields).  L.cap is the layout's positions per tile (chunks per tile * 64).
 // gs: the gather+scatter kernel, which keeps srb_lds_fields<T>(kf) fields of
-// a position in LDS (the gather and scatter kernels keep all kf).
+// a position in LDS (the gather and scatter kernels keep one).
 template <bool WITH_OP, typename F>
 static int srb_shape(const Launch &L, int op, F &&f, bool gs = false) {
   Launch W = L;
   const bool wave = L.variant == 2 && L.cap <= 16 * 64;
   const int lanes = wave ? 64 : kThreads;
   auto lds_of = [&](int kf) {
-    const int lf = gs ? srb_lds_fields<T>(kf) : kf;
+    const int lf = gs ? srb_lds_fields<T>(kf) : 1;
     return lds_bytes(W.cap, lf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short);
   };
   auto fits = [&](int kf) {

@@ -417,13 +417,16 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
 }
 
 // gather (Trans) on the sorted-run layout: row-leading positions fold their
-// row along the links and write gv at the row's CSR index
+// row along the links and write gv at the row's CSR index.  Several fields
+// pass through LDS one at a time (one field's LDS whatever KF); each lane
+// loads its position's fields, and a row-leading lane stores its row's
+// fields, as one wide access.
 template <typename T, int OP, int KF, int PER, int NT = kThreads>
 __global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const T *__restrict__ v,
                                                    T *__restrict__ gv) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
-  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
+  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap);
   const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
   const int kb = blockIdx.y * KF;
   const int cap = L.cap;
@@ -440,35 +443,41 @@ __global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const T
   for (int j = 0; j < PER; ++j)
 #pragma unroll
     for (int f = 0; f < KF; ++f) x[j][f] = v[(int64_t)id[j] * K + kb + f];
-  __syncthreads();
+  __syncthreads();  // next[] initialised before the links go in
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if ((valid >> j) & 1u) {
+    if (((valid >> j) & 1u) && prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
 #pragma unroll
-      for (int f = 0; f < KF; ++f) vals[(t + j * NT) * KF + f] = x[j][f];
-      if (prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
-    }
-  __syncthreads();
+  for (int f = 0; f < KF; ++f) {
+    if (f > 0) __syncthreads();  // the previous field's folds are done
 #pragma unroll
-  for (int j = 0; j < PER; ++j)
-    if (((valid >> j) & 1u) && prev[j] < 0) {
+    for (int j = 0; j < PER; ++j)
+      if ((valid >> j) & 1u) vals[t + j * NT] = x[j][f];
+    __syncthreads();
 #pragma unroll
-      for (int f = 0; f < KF; ++f) {
+    for (int j = 0; j < PER; ++j)
+      if (((valid >> j) & 1u) && prev[j] < 0) {
         T acc = Fold<T, OP>::init();
-        for (int i = t + j * NT, s = 0; i < cap && s < cap; i = next[i], ++s) Fold<T, OP>::apply(acc, vals[i * KF + f]);
-        gv[(int64_t)row[j] * K + kb + f] = acc;
+        for (int i = t + j * NT, s = 0; i < cap && s < cap; i = next[i], ++s) Fold<T, OP>::apply(acc, vals[i]);
+        x[j][f] = acc;  // a row-leading lane's value of field f is no longer needed
       }
-    }
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (((valid >> j) & 1u) && prev[j] < 0)
+#pragma unroll
+      for (int f = 0; f < KF; ++f) gv[(int64_t)row[j] * K + kb + f] = x[j][f];
 }
 
 // scatter (NoTrans) on the sorted-run layout: row-leading positions read
-// their row's value and pass it along the links; every position stores
+// their row's value and pass it along the links; every position stores.
+// Several fields pass through LDS one at a time, as in the gather.
 template <typename T, int KF, int PER, int NT = kThreads>
 __global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const T *__restrict__ gv,
                                                     T *__restrict__ v) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
-  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * KF);
+  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap);
   const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
   const int kb = blockIdx.y * KF;
   const int cap = L.cap;
@@ -492,19 +501,23 @@ __global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const 
 #pragma unroll
   for (int j = 0; j < PER; ++j)
     if (((valid >> j) & 1u) && prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
-  __syncthreads();
 #pragma unroll
-  for (int j = 0; j < PER; ++j)
-    if (((valid >> j) & 1u) && prev[j] < 0)
+  for (int f = 0; f < KF; ++f) {
+    __syncthreads();  // links in (f == 0) / the previous field read back
 #pragma unroll
-      for (int f = 0; f < KF; ++f)
-        for (int i = t + j * NT, s = 0; i < cap && s < cap; i = next[i], ++s) vals[i * KF + f] = g[j][f];
-  __syncthreads();
+    for (int j = 0; j < PER; ++j)
+      if (((valid >> j) & 1u) && prev[j] < 0)
+        for (int i = t + j * NT, s = 0; i < cap && s < cap; i = next[i], ++s) vals[i] = g[j][f];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if ((valid >> j) & 1u) g[j][f] = vals[t + j * NT];  // every position's value of field f
+  }
 #pragma unroll
   for (int j = 0; j < PER; ++j)
     if ((valid >> j) & 1u)
 #pragma unroll
-      for (int f = 0; f < KF; ++f) v[(int64_t)id[j] * K + kb + f] = vals[(t + j * NT) * KF + f];
+      for (int f = 0; f < KF; ++f) v[(int64_t)id[j] * K + kb + f] = g[j][f];
 }
 
 // ------------------------------------------------------------ general GS --
