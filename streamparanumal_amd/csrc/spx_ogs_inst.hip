@@ -59,10 +59,9 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
 }
 
 // sorted-run kernels: one-wave tiles (variant 2, up to 1024 positions) or 256
-// lanes.  All K fields go in one pass when the tile's registers and LDS hold
-// them; otherwise half of them per pass (K even), else one per pass (e.g.
-// FP64, K = 4 on 2048-position tiles: 64 KB of values, two passes of two
-// fields).  L.cap is the layout's positions per tile (chunks per tile * 64).
+// lanes.  All K fields go in one pass when the lanes' registers (pick_per)
+// and the LDS hold them; otherwise half of them per pass (K even), else one
+// per pass.  L.cap is the layout's positions per tile (chunks per tile * 64).
 // gs: the gather+scatter kernel, which keeps srb_lds_fields<T>(kf) fields of
 // a position in LDS (the gather and scatter kernels keep one).
 template <bool WITH_OP, typename F>
