@@ -244,9 +244,9 @@ struct SrbArgs {
 
 constexpr unsigned short kEnd = 0xffff;
 
-// The layout arrays as restrict kernel arguments: the compiler may then read
-// the wave-uniform chunk records through the scalar cache although the
-// kernel stores to its value arrays.
+// The layout arrays as restrict kernel arguments: they never alias the value
+// arrays the kernels store to, so the compiler may keep their loads ahead of
+// those stores.
 #define SRB_PARAMS \
   const SrbChunk *__restrict__ chunks, const int *__restrict__ runIds, const unsigned short *__restrict__ links
 #define SRB_ARGS(A) A.chunks, A.runIds, A.links
