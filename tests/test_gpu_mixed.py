@@ -36,7 +36,7 @@ def setups(dev, oracle):
 
 @pytest.mark.parametrize("mesh", MESHES)
 @pytest.mark.parametrize("st", STORES)
-@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("sorted_runs", [True, False])
 def test_gs_mixed_bit_exact(setups, oracle, mesh, st, k, sorted_runs):
     say("gs mixed", mesh, st, k, "sorted-run" if sorted_runs else "csr")
