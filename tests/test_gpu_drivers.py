@@ -95,3 +95,11 @@ def test_launch_sweep():
     assert r.returncode == 0, r.stderr[-2000:]
     rows = sweep_rows(r.stdout, "BS0")
     assert [int(x[0]) for x in rows] == [100, 200, 300, 400]
+
+
+@pytest.mark.parametrize("name", [f"BS{i}" for i in range(9)])
+def test_runbs_invocation(name):
+    """BS/runBS.sh runs every driver as `BSn -m HIP` with its default settings."""
+    r = run(name, "-m", "HIP")
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert f"{name} = [" in r.stdout
