@@ -244,9 +244,37 @@ int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, vo
 /* gather (ogsKernels.okl:86-122) and scatter (ogsKernels.okl:127-164) on the
    sorted-run layout of a CSR: same results as spx_ogs_gather /
    spx_ogs_scatter on that CSR; the row of a row-leading position is its
-   record's chunkRow plus the row-leading positions before it in the record */
+   record's chunkRow plus the row-leading positions before it in the record,
+   so each tile's rows must ascend by their first member's local index
+   (spxh_srb_rows_ascending) */
 int spx_ogs_gather_srb(int type, int op, int K, const spx_srb_t *srb, const void *v, void *gv);
 int spx_ogs_scatter_srb(int type, int K, const spx_srb_t *srb, const void *gv, void *v);
+/* The N-CSR forms of an ogs::Signed operator on the sorted-run layout of its
+   T-CSR.  The N-CSR's row r holds the members of T-CSR row r with a positive
+   id, in the same order (ogsOperator.cpp:621-643); memberN has one mask per
+   chunk record (index b*chunks_per_tile + k, like the records), bit l set when
+   position l is such a member.  The layout must satisfy that: N-CSR row r is
+   the set members of T-CSR row r for r < nrowsN, and later rows have none
+   (spxh_srb_build_signed checks it).
+     gather_scatter, SPX_NOTRANS: each row folds its N members (the op's
+       identity when it has none) and stores to every member; SPX_TRANS: each
+       row folds every member and stores to its N members only; SPX_SYM:
+       spx_ogs_gather_scatter_srb, memberN unused (ogsOperator.cpp:340-409,
+       442-481, the gatherScatter kernel on gather N / scatter T and on gather
+       T / scatter N);
+     gather: the NoTrans gather, gv[r] for r < nrowsN over the N members of
+       row r (ogsOperator.cpp:176-204, N-CSR);
+     scatter: the Trans scatter, v[m] = gv[r] for the N members m of rows
+       r < nrowsN, other entries untouched (ogsOperator.cpp:295-322, N-CSR);
+       gather and scatter number rows as spx_ogs_gather_srb does, so they
+       need the same row order (spxh_srb_rows_ascending).
+   Results are bit-identical to the CSR kernels on the N- and T-CSR. */
+int spx_ogs_gather_scatter_srb_signed(int type, int op, int trans, int K, const spx_srb_t *srb,
+                                      const uint64_t *memberN, void *q);
+int spx_ogs_gather_srb_signed(int type, int op, int K, const spx_srb_t *srb, const uint64_t *memberN,
+                              int32_t nrowsN, const void *v, void *gv);
+int spx_ogs_scatter_srb_signed(int type, int K, const spx_srb_t *srb, const uint64_t *memberN, int32_t nrowsN,
+                               const void *gv, void *v);
 
 /* ------------------------------------------- reduced-precision storage -- */
 /* Storage types of the mixed-precision variants (no reference counterpart:

@@ -142,6 +142,22 @@ int spxh_srb_build(int32_t nrows, const int32_t *rowStarts, const int32_t *colId
 int spxh_srb_sizes(spxh_srb_t srb, int64_t *sizes);
 int spxh_srb_copy(spxh_srb_t srb, spx_srb_chunk_t *chunks, int32_t *runIds, uint16_t *links, int32_t *entryStarts);
 int spxh_srb_destroy(spxh_srb_t srb);
+/* 1 when every tile's rows ascend by their first member's local index, as
+   the layout's gather and scatter (which number rows by their leading
+   positions) need; 0 otherwise, e.g. for the T-CSR of an ogs::Signed
+   operator without unique flagging, whose groups with no positive member
+   follow the others (ogsSetup.cpp:517-546).  The gather+scatter forms need
+   no row numbers and run on any layout. */
+int spxh_srb_rows_ascending(spxh_srb_t srb, int32_t *ascending);
+/* The same for the T-CSR of an ogs::Signed operator (ncols local nodes)
+   together with its N-CSR (ogsOperator.cpp:621-643), which must hold the
+   members of the same T-CSR rows, in the same order, for its nrowsN rows
+   (refused otherwise); spxh_srb_copy_members then fills the nchunks N-member
+   masks of spx_ogs_*_srb_signed. */
+int spxh_srb_build_signed(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int32_t nblocks,
+                          const int32_t *blockStarts, int32_t max_block_nnz, int32_t ncols, int32_t nrowsN,
+                          const int32_t *rowStartsN, const int32_t *colIdsN, spxh_srb_t *out);
+int spxh_srb_copy_members(spxh_srb_t srb, uint64_t *memberN);
 
 /* mesh_t hex box ids: lexicographic GLL-grid ids of an nx x ny x nz box at
    degree p, n = nx*ny*nz*(p+1)^3 entries (meshSetupBoxHex3D.cpp:87-130) */

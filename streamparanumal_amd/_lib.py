@@ -127,6 +127,9 @@ SPX_FUNCS = {
     "spx_ogs_gather_scatter_srb_mixed": (C.c_int, [C.c_int, C.c_int, C.POINTER(Srb), vp]),
     "spx_ogs_gather_srb": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Srb), vp, vp]),
     "spx_ogs_scatter_srb": (C.c_int, [C.c_int, C.c_int, C.POINTER(Srb), vp, vp]),
+    "spx_ogs_gather_scatter_srb_signed": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Srb), vp, vp]),
+    "spx_ogs_gather_srb_signed": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(Srb), vp, C.c_int32, vp, vp]),
+    "spx_ogs_scatter_srb_signed": (C.c_int, [C.c_int, C.c_int, C.POINTER(Srb), vp, C.c_int32, vp, vp]),
     "spx_ogs_gather_scatter_mixed": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, i32, vp]),
     "spx_cg_plan_create": (C.c_int, [i32, vp, i32, C.POINTER(vp)]),
     "spx_cg_plan_destroy": (C.c_int, [vp]),
@@ -163,6 +166,9 @@ SPXH_FUNCS = {
     "spxh_srb_sizes": (C.c_int, [vp, vp]),
     "spxh_srb_copy": (C.c_int, [vp, vp, vp, vp, vp]),
     "spxh_srb_destroy": (C.c_int, [vp]),
+    "spxh_srb_build_signed": (C.c_int, [i32, vp, vp, i32, vp, i32, i32, i32, vp, vp, C.POINTER(vp)]),
+    "spxh_srb_copy_members": (C.c_int, [vp, vp]),
+    "spxh_srb_rows_ascending": (C.c_int, [vp, vp]),
     "spxh_ogs_gather": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_ogs_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_csr_build": (C.c_int, [i32, vp, C.c_int, C.c_int, C.POINTER(vp)]),

@@ -133,10 +133,10 @@ int launch_gs_srb_mixed(const Launch &L, const SrbArgs &A, S *q) {
   do {                                                                                                     \
     if (wave)                                                                                              \
       hipLaunchKernelGGL((k_gs_srb<S, SPX_ADD, KF, PER, 64, FoldWide<S>>), grid, dim3(64), lds, W.stream, W, \
-                         SRB_ARGS(A), q);                                                                  \
+                         SRB_ARGS(A), q, nullptr);                                                         \
     else                                                                                                   \
       hipLaunchKernelGGL((k_gs_srb<S, SPX_ADD, KF, PER, kThreads, FoldWide<S>>), grid, dim3(kThreads), lds,  \
-                         W.stream, W, SRB_ARGS(A), q);                                                     \
+                         W.stream, W, SRB_ARGS(A), q, nullptr);                                            \
   } while (0)
   if (W.kf == 1 && W.per == 4) SPX_GSRM(1, 4);
   else if (W.kf == 1 && W.per == 8) SPX_GSRM(1, 8);

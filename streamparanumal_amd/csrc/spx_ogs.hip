@@ -184,7 +184,7 @@ int spx_ogs_gather_scatter_srb(int type, int op, int K, const spx_srb_t *srb, vo
   if (!q) return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb: null pointer");
   rc = by_type(type, [&](auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
-    return launch_gs_srb(L, op, A, (T *)q);
+    return launch_gs_srb(L, op, A, (T *)q, kAll, nullptr);
   });
   if (rc) return rc;
   SPX_LAUNCHED("spx_ogs_gather_scatter_srb");
@@ -201,7 +201,7 @@ int spx_ogs_gather_srb(int type, int op, int K, const spx_srb_t *srb, const void
   if (!v || !gv) return fail(SPX_ERR_ARG, "spx_ogs_gather_srb: null pointer");
   rc = by_type(type, [&](auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
-    return launch_gather_srb(L, op, A, (const T *)v, (T *)gv);
+    return launch_gather_srb(L, op, A, (const T *)v, (T *)gv, kAll, nullptr);
   });
   if (rc) return rc;
   SPX_LAUNCHED("spx_ogs_gather_srb");
@@ -218,10 +218,76 @@ int spx_ogs_scatter_srb(int type, int K, const spx_srb_t *srb, const void *gv, v
   if (!v || !gv) return fail(SPX_ERR_ARG, "spx_ogs_scatter_srb: null pointer");
   rc = by_type(type, [&](auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
-    return launch_scatter_srb(L, A, (const T *)gv, (T *)v);
+    return launch_scatter_srb(L, A, (const T *)gv, (T *)v, kAll, nullptr);
   });
   if (rc) return rc;
   SPX_LAUNCHED("spx_ogs_scatter_srb");
+  return SPX_OK;
+}
+
+int spx_ogs_gather_scatter_srb_signed(int type, int op, int trans, int K, const spx_srb_t *srb,
+                                      const uint64_t *memberN, void *q) {
+  SPX_NEED_DEVICE();
+  Launch L;
+  SrbArgs A;
+  const char *name = "spx_ogs_gather_scatter_srb_signed";
+  int rc = srb_launch_setup(name, type, K, srb, L, A);
+  if (rc || (rc = check_op(name, op))) return rc;
+  if (trans != SPX_SYM && trans != SPX_NOTRANS && trans != SPX_TRANS)
+    return fail(SPX_ERR_ARG, std::string(name) + ": unknown transpose");
+  if (srb->nblocks == 0) return SPX_OK;
+  if (!q || (trans != SPX_SYM && !memberN)) return fail(SPX_ERR_ARG, std::string(name) + ": null pointer");
+  const int mode = trans == SPX_NOTRANS ? kFoldN : trans == SPX_TRANS ? kStoreN : kAll;
+  rc = by_type(type, [&](auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return launch_gs_srb(L, op, A, (T *)q, mode, reinterpret_cast<const unsigned long long *>(memberN));
+  });
+  if (rc) return rc;
+  SPX_LAUNCHED(name);
+  return SPX_OK;
+}
+
+int spx_ogs_gather_srb_signed(int type, int op, int K, const spx_srb_t *srb, const uint64_t *memberN,
+                              int32_t nrowsN, const void *v, void *gv) {
+  SPX_NEED_DEVICE();
+  Launch L;
+  SrbArgs A;
+  const char *name = "spx_ogs_gather_srb_signed";
+  int rc = srb_launch_setup(name, type, K, srb, L, A);
+  if (rc || (rc = check_op(name, op))) return rc;
+  if (nrowsN < 0) return fail(SPX_ERR_ARG, std::string(name) + ": negative row count");
+  if (srb->nblocks == 0 || nrowsN == 0) return SPX_OK;
+  if (!memberN || !v || !gv) return fail(SPX_ERR_ARG, std::string(name) + ": null pointer");
+  L.rows = nrowsN;
+  rc = by_type(type, [&](auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return launch_gather_srb(L, op, A, (const T *)v, (T *)gv, kFoldN,
+                             reinterpret_cast<const unsigned long long *>(memberN));
+  });
+  if (rc) return rc;
+  SPX_LAUNCHED(name);
+  return SPX_OK;
+}
+
+int spx_ogs_scatter_srb_signed(int type, int K, const spx_srb_t *srb, const uint64_t *memberN, int32_t nrowsN,
+                               const void *gv, void *v) {
+  SPX_NEED_DEVICE();
+  Launch L;
+  SrbArgs A;
+  const char *name = "spx_ogs_scatter_srb_signed";
+  int rc = srb_launch_setup(name, type, K, srb, L, A);
+  if (rc) return rc;
+  if (nrowsN < 0) return fail(SPX_ERR_ARG, std::string(name) + ": negative row count");
+  if (srb->nblocks == 0 || nrowsN == 0) return SPX_OK;
+  if (!memberN || !v || !gv) return fail(SPX_ERR_ARG, std::string(name) + ": null pointer");
+  L.rows = nrowsN;
+  rc = by_type(type, [&](auto *tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    return launch_scatter_srb(L, A, (const T *)gv, (T *)v, kStoreN,
+                              reinterpret_cast<const unsigned long long *>(memberN));
+  });
+  if (rc) return rc;
+  SPX_LAUNCHED(name);
   return SPX_OK;
 }
 

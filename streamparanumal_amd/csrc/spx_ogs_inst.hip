@@ -89,40 +89,63 @@ static int srb_shape(const Launch &L, int op, F &&f, bool gs = false) {
   });
 }
 
-int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q) {
+// mode (kAll, kFoldN, kStoreN) and memberN: the N-CSR forms, on 256-lane
+// tiles only (one instance set per mode).
+int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q, int mode, const unsigned long long *memberN) {
+  Launch L2 = L;
+  if (mode != kAll) L2.variant = 0;
   return srb_shape<true>(
-      L, op,
+      L2, op,
       [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
-        if (wave)
+        if (mode == kFoldN)
+          hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, kThreads, Fold<T, OPC>, kFoldN>), grid_of(W),
+                             dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), q, memberN);
+        else if (mode == kStoreN)
+          hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, kThreads, Fold<T, OPC>, kStoreN>), grid_of(W),
+                             dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), q, memberN);
+        else if (wave)
           hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A),
-                             q);
+                             q, nullptr);
         else
           hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W,
-                             SRB_ARGS(A), q);
+                             SRB_ARGS(A), q, nullptr);
         return SPX_OK;
       },
       true);
 }
 
-int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const T *v, T *gv) {
-  return srb_shape<true>(L, op, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
-    if (wave)
+int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const T *v, T *gv, int mode,
+                      const unsigned long long *memberN) {
+  Launch L2 = L;
+  if (mode != kAll) L2.variant = 0;
+  return srb_shape<true>(L2, op, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
+    if (mode == kFoldN)
+      hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER, kThreads, kFoldN>), grid_of(W), dim3(kThreads), lds,
+                         W.stream, W, SRB_ARGS(A), v, gv, memberN);
+    else if (wave)
       hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A), v,
-                         gv);
+                         gv, nullptr);
     else
       hipLaunchKernelGGL((k_gather_srb<T, OPC, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), v,
-                         gv);
+                         gv, nullptr);
     return SPX_OK;
   });
 }
 
-int launch_scatter_srb(const Launch &L, const SrbArgs &A, const T *gv, T *v) {
-  return srb_shape<false>(L, SPX_ADD, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
-    if (wave)
-      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A), gv, v);
+int launch_scatter_srb(const Launch &L, const SrbArgs &A, const T *gv, T *v, int mode,
+                       const unsigned long long *memberN) {
+  Launch L2 = L;
+  if (mode != kAll) L2.variant = 0;
+  return srb_shape<false>(L2, SPX_ADD, [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
+    if (mode == kStoreN)
+      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER, kThreads, kStoreN>), grid_of(W), dim3(kThreads), lds, W.stream,
+                         W, SRB_ARGS(A), gv, v, memberN);
+    else if (wave)
+      hipLaunchKernelGGL((k_scatter_srb<T, KF, PER, 64>), grid_of(W), dim3(64), lds, W.stream, W, SRB_ARGS(A), gv, v,
+                         nullptr);
     else
       hipLaunchKernelGGL((k_scatter_srb<T, KF, PER>), grid_of(W), dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), gv,
-                         v);
+                         v, nullptr);
     return SPX_OK;
   });
 }

@@ -27,6 +27,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <type_traits>
 
 #include "spx_internal.hpp"
 
@@ -48,6 +49,7 @@ struct Launch {
                 // 3 = 16-bit row starts in LDS
   hipStream_t stream;
   int *err;
+  int rows = 0;  // sorted-run N-CSR gather / scatter: rows of the N-CSR
 };
 
 template <typename T> struct Lim;
@@ -302,12 +304,48 @@ __device__ __forceinline__ void srb_decode(const SrbChunk *__restrict__ chunks, 
   }
 }
 
+// Which members of a row take part.  An ogs::Signed operator's N-CSR row r
+// holds the members of T-CSR row r whose id is positive, in the same order
+// (ogsOperator.cpp:621-643), so on a layout of the T-CSR whose positions carry
+// an N-membership bit: kAll uses every member (Sym; the T-CSR gather and
+// scatter), kFoldN folds over the N members only (NoTrans gather+scatter, the
+// N-CSR gather) and kStoreN stores to the N members only (Trans
+// gather+scatter, the N-CSR scatter).  A row without N members folds to the
+// op's identity.
+enum { kAll = 0, kFoldN = 1, kStoreN = 2 };
+
+// Bit j: position j*NT + threadIdx.x of tile b is an N member, read from
+// memberN[record] (bit l: position l of the record).  Lane l loads dword l%2
+// of the mask of record l/2 (one coalesced load for up to 32 records); the
+// masks then reach every lane by readlane.  Records past the tile's cpt are
+// replaced by its last one, as in srb_decode, whose validity bits mask them.
+template <int PER, int NT>
+__device__ __forceinline__ unsigned srb_members(const unsigned long long *__restrict__ memberN, int b, int cpt) {
+  static_assert(PER <= 32, "one load covers 32 records");
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int l = t & 63;
+  const unsigned *words = reinterpret_cast<const unsigned *>(memberN);
+  const int jr = min(l >> 1, PER - 1);
+  const int kc = min((NT / 64) * jr + w, cpt - 1);
+  const unsigned word = words[((int64_t)b * cpt + kc) * 2 + (l & 1)];
+  unsigned bits = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)word, 2 * j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)word, 2 * j + 1);
+    bits |= (((l < 32 ? lo : hi) >> (l & 31)) & 1u) << j;
+  }
+  return bits;
+}
+
 // Values a sorted-run gather+scatter tile keeps in LDS per position: every
 // field for 2-byte values (word-major, below), one field at a time otherwise.
 template <typename T> constexpr int srb_lds_fields(int kf) { return (kf > 1 && sizeof(T) == 2) ? kf : 1; }
 
-template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>>
-__global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restrict__ q) {
+template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>, int MODE = kAll>
+__global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restrict__ q,
+                                               const unsigned long long *__restrict__ memberN) {
   // Several 4- or 8-byte fields per pass: each lane loads and stores its
   // position's KF values as one wide access (the compiler merges them) and
   // keeps them in registers, and the fields pass through LDS one at a time,
@@ -321,7 +359,14 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
   // position's lane.  One access then spans a run's consecutive words instead
   // of KF separate 2-byte accesses per lane (BF16, K = 3: 13 % faster), and
   // the LDS slot of word w is the chunk's base slot + w.
+  //
+  // MODE kFoldN / kStoreN (memberN given): positions outside the N-CSR hold
+  // the op's identity in the fold (an exact no-op: the sum starts from +0 and
+  // so is never -0, x*1 == x, and the max / min never pass -/+ the type's
+  // largest value) or keep their value.
   constexpr bool kWord = KF > 1 && sizeof(T) == 2;
+  static_assert(MODE == kAll || (!kWord && std::is_same<typename F::Acc, T>::value),
+                "N-CSR forms fold plain 4- and 8-byte values");
   constexpr int LF = kWord ? KF : 1;  // fields held in LDS per position
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
@@ -338,6 +383,14 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
   unsigned valid;
   srb_decode<PER, NT>(chunks, runIds, links, b, cap / 64, id, prev, row, valid);
   (void)row;
+  // positions whose value enters the fold (ld) and that store the result (st)
+  unsigned ld = valid, st = valid;
+  if constexpr (MODE != kAll) {
+    const unsigned m = valid & srb_members<PER, NT>(memberN, b, cap / 64);
+    if constexpr (MODE == kFoldN) ld = m;
+    else st = m;
+  }
+  (void)ld;
   const int l = t & 63;
   const int cb0 = t - l;  // first position of this wave's chunk 0
   T v[PER][KF];
@@ -346,7 +399,8 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
 #pragma unroll
     for (int f = 0; f < KF; ++f) {
       if constexpr (!kWord) {
-        v[j][f] = q[(int64_t)id[j] * K + kb + f];
+        if constexpr (MODE == kFoldN) v[j][f] = ((ld >> j) & 1u) ? q[(int64_t)id[j] * K + kb + f] : F::init();
+        else v[j][f] = q[(int64_t)id[j] * K + kb + f];
       } else {
         const int w = f * 64 + l, src = w / KF;
         const int idw = __shfl(id[j], src, 64);
@@ -399,9 +453,9 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     if constexpr (KF == 1) {
-      if ((valid >> j) & 1u) q[(int64_t)id[j] * K + kb] = vals[t + j * NT];
+      if ((st >> j) & 1u) q[(int64_t)id[j] * K + kb] = vals[t + j * NT];
     } else if constexpr (!kWord) {
-      if ((valid >> j) & 1u)
+      if ((st >> j) & 1u)
 #pragma unroll
         for (int f = 0; f < KF; ++f) q[(int64_t)id[j] * K + kb + f] = v[j][f];
     } else {
@@ -420,10 +474,14 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
 // row along the links and write gv at the row's CSR index.  Several fields
 // pass through LDS one at a time (one field's LDS whatever KF); each lane
 // loads its position's fields, and a row-leading lane stores its row's
-// fields, as one wide access.
-template <typename T, int OP, int KF, int PER, int NT = kThreads>
+// fields, as one wide access.  MODE kFoldN gathers the N-CSR (NoTrans): only
+// N members load (the others hold the op's identity) and only rows below
+// L.rows store.
+template <typename T, int OP, int KF, int PER, int NT = kThreads, int MODE = kAll>
 __global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const T *__restrict__ v,
-                                                   T *__restrict__ gv) {
+                                                   T *__restrict__ gv,
+                                                   const unsigned long long *__restrict__ memberN) {
+  static_assert(MODE == kAll || MODE == kFoldN, "gather modes");
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
   unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap);
@@ -438,11 +496,17 @@ __global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const T
   int id[PER], prev[PER], row[PER];
   unsigned valid;
   srb_decode<PER, NT>(chunks, runIds, links, b, cap / 64, id, prev, row, valid);
+  unsigned ld = valid;
+  if constexpr (MODE == kFoldN) ld = valid & srb_members<PER, NT>(memberN, b, cap / 64);
+  (void)ld;
   T x[PER][KF];
 #pragma unroll
   for (int j = 0; j < PER; ++j)
 #pragma unroll
-    for (int f = 0; f < KF; ++f) x[j][f] = v[(int64_t)id[j] * K + kb + f];
+    for (int f = 0; f < KF; ++f) {
+      if constexpr (MODE == kFoldN) x[j][f] = ((ld >> j) & 1u) ? v[(int64_t)id[j] * K + kb + f] : Fold<T, OP>::init();
+      else x[j][f] = v[(int64_t)id[j] * K + kb + f];
+    }
   __syncthreads();  // next[] initialised before the links go in
 #pragma unroll
   for (int j = 0; j < PER; ++j)
@@ -464,17 +528,21 @@ __global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const T
   }
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (((valid >> j) & 1u) && prev[j] < 0)
+    if (((valid >> j) & 1u) && prev[j] < 0 && (MODE == kAll || row[j] < L.rows))
 #pragma unroll
       for (int f = 0; f < KF; ++f) gv[(int64_t)row[j] * K + kb + f] = x[j][f];
 }
 
 // scatter (NoTrans) on the sorted-run layout: row-leading positions read
 // their row's value and pass it along the links; every position stores.
-// Several fields pass through LDS one at a time, as in the gather.
-template <typename T, int KF, int PER, int NT = kThreads>
+// Several fields pass through LDS one at a time, as in the gather.  MODE
+// kStoreN scatters the N-CSR (Trans): only rows below L.rows read and only N
+// members store.
+template <typename T, int KF, int PER, int NT = kThreads, int MODE = kAll>
 __global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const T *__restrict__ gv,
-                                                    T *__restrict__ v) {
+                                                    T *__restrict__ v,
+                                                    const unsigned long long *__restrict__ memberN) {
+  static_assert(MODE == kAll || MODE == kStoreN, "scatter modes");
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
   unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap);
@@ -489,6 +557,8 @@ __global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const 
   int id[PER], prev[PER], row[PER];
   unsigned valid;
   srb_decode<PER, NT>(chunks, runIds, links, b, cap / 64, id, prev, row, valid);
+  unsigned st = valid;
+  if constexpr (MODE == kStoreN) st = valid & srb_members<PER, NT>(memberN, b, cap / 64);
   // only row-leading positions read (the row index of the others may lie
   // one past the last row)
   T g[PER][KF];
@@ -496,7 +566,9 @@ __global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const 
   for (int j = 0; j < PER; ++j)
 #pragma unroll
     for (int f = 0; f < KF; ++f)
-      g[j][f] = (((valid >> j) & 1u) && prev[j] < 0) ? gv[(int64_t)row[j] * K + kb + f] : T(0);
+      g[j][f] = (((valid >> j) & 1u) && prev[j] < 0 && (MODE == kAll || row[j] < L.rows))
+                    ? gv[(int64_t)row[j] * K + kb + f]
+                    : T(0);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j)
@@ -515,7 +587,7 @@ __global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const 
   }
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if ((valid >> j) & 1u)
+    if ((st >> j) & 1u)
 #pragma unroll
       for (int f = 0; f < KF; ++f) v[(int64_t)id[j] * K + kb + f] = g[j][f];
 }
@@ -774,9 +846,12 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
   int launch_scatter(const Launch &L, const int *bs, const int *st, const int *ids, const T *gv, T *v);    \
   int launch_extract(hipStream_t s, int64_t total, int K, const int *ids, const T *q, T *gq);           \
   int launch_scatter_inv(hipStream_t s, int64_t n, int K, const int *g2l, const T *gv, T *v, int npl); \
-  int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q);                                       \
-  int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const T *v, T *gv);                      \
-  int launch_scatter_srb(const Launch &L, const SrbArgs &A, const T *gv, T *v);
+  int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q, int mode,                              \
+                    const unsigned long long *memberN);                                                     \
+  int launch_gather_srb(const Launch &L, int op, const SrbArgs &A, const T *v, T *gv, int mode,             \
+                        const unsigned long long *memberN);                                                 \
+  int launch_scatter_srb(const Launch &L, const SrbArgs &A, const T *gv, T *v, int mode,                    \
+                         const unsigned long long *memberN);
 SPX_OGS_GEN_DECL(float)
 SPX_OGS_GEN_DECL(double)
 SPX_OGS_GEN_DECL(int)

@@ -26,25 +26,47 @@ typedef spx_srb_chunk_t SrbChunkRecord;
 
 struct SortedRunBlocks {
   dlong nblocks = 0, cpt = 0, nnz = 0, nchunks = 0, nruns = 0, nlinks = 0;
+  // Row-leading positions number the rows in position order, so the gather
+  // and scatter (which use row numbers) need each tile's rows to ascend by
+  // their first member's local index.  True of every operator's T-CSR except
+  // a Signed one without unique flagging, whose groups with no positive member
+  // follow the others (ogsSetup.cpp:517-546); the gather+scatter, which needs
+  // no row numbers, runs on any layout.
+  bool rowsAscending = true;
   int xcdRemap = -1;  // launch hints passed in spx_srb_t (-1 / 0: library default)
   int tileLanes = 0;
   memory<dlong> entryStarts;  // host only: CSR entries of each tile
   memory<SrbChunkRecord> chunks;
   memory<dlong> runIds;
   memory<uint16_t> links;
+  // N-CSR membership, one mask per chunk record (bit l: position l is an
+  // N-CSR member), when built with member flags; empty otherwise
+  memory<uint64_t> memberN;
   deviceMemory<SrbChunkRecord> o_chunks;
   deviceMemory<dlong> o_runIds;
   deviceMemory<uint16_t> o_links;
+  deviceMemory<uint64_t> o_memberN;
 
   void Upload(platform_t &platform);
   spx_srb_t Device() const;
+  const uint64_t *DeviceMemberN() const { return o_memberN.length() ? o_memberN.ptr() : nullptr; }
 };
 
 // chunk records per tile for a row block capacity (a power of two >= 4)
 int SrbChunksPerTile(int cap);
 
+// memberFlag (one byte per local node, optional): fills memberN.
 SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowStarts, const memory<dlong> colIds,
-                                     const dlong Nblocks, const memory<dlong> blockStarts, const int cap);
+                                     const dlong Nblocks, const memory<dlong> blockStarts, const int cap,
+                                     const uint8_t *memberFlag = nullptr);
+
+// Per local node (Ncols), 1 when it is an N-CSR member, if every N-CSR row r
+// is T-CSR row r restricted to those members (same order) and T-CSR rows past
+// the N-CSR's have none -- true of one rank's Setup (ogsOperator.cpp:621-643)
+// and of a batch whose parts' N rows match their T rows; empty otherwise.
+memory<uint8_t> SignedMemberFlags(const dlong Ncols, const dlong NrowsT, const memory<dlong> rowStartsT,
+                                  const memory<dlong> colIdsT, const dlong NrowsN, const memory<dlong> rowStartsN,
+                                  const memory<dlong> colIdsN);
 
 // One launch shape of the Sym gather+scatter and its measured time.
 struct GsTuning {
@@ -104,6 +126,12 @@ class ogsOperator_t {
   // gather 6.8 against 5.6.
   bool gatherSorted = true;
   bool scatterSorted = true;
+  // Signed operators whose N-CSR rows are the positive members of the same
+  // T-CSR rows carry N-membership masks in gsSrb (signedSrb), and their
+  // NoTrans / Trans gather+scatter, N-CSR gather and N-CSR scatter run on
+  // the layout too (under gsSorted / gatherSorted / scatterSorted).
+  bool signedSrb = false;
+  memory<uint8_t> memberFlagN;  // per local node: N-CSR member (signedSrb only)
 
   // Local -> gathered row maps of the T- and N-CSR (-1: in no row), for the
   // scatter in local order (spx_ogs_scatter_inverse).  Measured on MI355X at

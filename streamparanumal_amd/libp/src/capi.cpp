@@ -389,6 +389,49 @@ int spxh_srb_build(int32_t nrows, const int32_t *rowStarts, const int32_t *colId
   });
 }
 
+int spxh_srb_build_signed(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int32_t nblocks,
+                          const int32_t *blockStarts, int32_t cap, int32_t ncols, int32_t nrowsN,
+                          const int32_t *rowStartsN, const int32_t *colIdsN, spxh_srb_t *out) {
+  if (!out || !rowStarts || !blockStarts || !rowStartsN || nrows < 0 || nblocks < 0 || ncols < 0 || nrowsN < 0)
+    return err(SPX_ERR_ARG, "spxh_srb_build_signed: bad arguments");
+  *out = nullptr;
+  return guarded([&] {
+    LIBP_ABORT("blockStarts must run from row 0 to row nrows",
+               blockStarts[0] != 0 || blockStarts[nblocks] != nrows);
+    for (int32_t b = 0; b < nblocks; ++b)
+      LIBP_ABORT("blockStarts must be nondecreasing", blockStarts[b + 1] < blockStarts[b]);
+    auto h = std::make_unique<spxh_srb_s>();
+    memory<dlong> rs(nrows + 1), bs(nblocks + 1), rsN(nrowsN + 1);
+    std::copy(rowStarts, rowStarts + nrows + 1, rs.ptr());
+    std::copy(blockStarts, blockStarts + nblocks + 1, bs.ptr());
+    std::copy(rowStartsN, rowStartsN + nrowsN + 1, rsN.ptr());
+    const dlong nnz = rs[nrows], nnzN = rsN[nrowsN];
+    LIBP_ABORT("colIds missing", (nnz > 0 && !colIds) || (nnzN > 0 && !colIdsN));
+    for (int32_t i = 0; i < nnz; ++i) LIBP_ABORT("colIds outside [0, ncols)", colIds[i] < 0 || colIds[i] >= ncols);
+    memory<dlong> ci(nnz > 0 ? nnz : 1), ciN(nnzN > 0 ? nnzN : 1);
+    if (nnz > 0) std::copy(colIds, colIds + nnz, ci.ptr());
+    if (nnzN > 0) std::copy(colIdsN, colIdsN + nnzN, ciN.ptr());
+    memory<uint8_t> flag = ogs::SignedMemberFlags(ncols, nrows, rs, ci, nrowsN, rsN, ciN);
+    LIBP_ABORT("the N-CSR's rows are not the flagged members of the same T-CSR rows", flag.length() == 0);
+    h->srb = ogs::BuildSortedRunBlocks(nrows, rs, ci, nblocks, bs, cap, flag.ptr());
+    *out = h.release();
+  });
+}
+
+int spxh_srb_rows_ascending(spxh_srb_t h, int32_t *ascending) {
+  if (!h || !ascending) return err(SPX_ERR_ARG, "spxh_srb_rows_ascending: null");
+  *ascending = h->srb.rowsAscending ? 1 : 0;
+  return SPX_OK;
+}
+
+int spxh_srb_copy_members(spxh_srb_t h, uint64_t *memberN) {
+  if (!h || !memberN) return err(SPX_ERR_ARG, "spxh_srb_copy_members: null");
+  const ogs::SortedRunBlocks &s = h->srb;
+  if (!s.memberN.length()) return err(SPX_ERR_ARG, "spxh_srb_copy_members: layout built without member flags");
+  if (s.nchunks) std::copy(s.memberN.ptr(), s.memberN.ptr() + s.nchunks, memberN);
+  return SPX_OK;
+}
+
 int spxh_srb_sizes(spxh_srb_t h, int64_t *sizes) {
   if (!h || !sizes) return err(SPX_ERR_ARG, "spxh_srb_sizes: null");
   const ogs::SortedRunBlocks &s = h->srb;

@@ -53,6 +53,10 @@ ogsOperator_t::ogsOperator_t(platform_t &platform_, Kind kind_, const dlong Nrow
     o_rowStartsN = o_rowStartsT;
     o_colIdsN = o_colIdsT;
   }
+  if (kind == Signed) {
+    memberFlagN = SignedMemberFlags(Ncols, NrowsT, rowStartsT, colIdsT, NrowsN, rowStartsN, colIdsN);
+    signedSrb = memberFlagN.length() > 0;
+  }
   SetBlockNnz(blockNnz_);
   SetGsBlockNnz(gsBlockNnz_);
   auto localRows = [&](dlong nrows, const memory<dlong> &rs, const memory<dlong> &ci) {
@@ -74,7 +78,8 @@ void ogsOperator_t::SetGsBlockNnz(int nnz) {
   blockRowsGreedy(NrowsT, rowStartsT, gsBlockNnz, NgsBlocks, gsBlockRowStarts);
   o_gsBlockRowStarts = platform.malloc<dlong>(NgsBlocks + 1, gsBlockRowStarts);
   const int xcd = gsSrb.xcdRemap, lanes = gsSrb.tileLanes;
-  gsSrb = BuildSortedRunBlocks(NrowsT, rowStartsT, colIdsT, NgsBlocks, gsBlockRowStarts, gsBlockNnz);
+  gsSrb = BuildSortedRunBlocks(NrowsT, rowStartsT, colIdsT, NgsBlocks, gsBlockRowStarts, gsBlockNnz,
+                               signedSrb ? memberFlagN.ptr() : nullptr);
   gsSrb.xcdRemap = xcd;
   gsSrb.tileLanes = lanes;
   gsSrb.Upload(platform);
@@ -107,11 +112,16 @@ void ogsOperator_t::SetBlockNnz(int nnz) {
 void ogsOperator_t::Free() { *this = ogsOperator_t(); }
 
 void ogsOperator_t::GatherRaw(Type type, Op op, Transpose trans, int k, const void *v, void *gv) {
-  if (trans == NoTrans) {
+  const bool srbRows = NgsBlocks && gsSrb.rowsAscending;  // the layout numbers the rows
+  if (trans == NoTrans && gatherSorted && srbRows && (signedSrb || kind == Unsigned)) {
+    const spx_srb_t d = gsSrb.Device();
+    if (kind == Unsigned) SPX_CHECK(spx_ogs_gather_srb((int)type, (int)op, k, &d, v, gv));
+    else SPX_CHECK(spx_ogs_gather_srb_signed((int)type, (int)op, k, &d, gsSrb.DeviceMemberN(), NrowsN, v, gv));
+  } else if (trans == NoTrans) {
     if (NrowBlocksN)
       SPX_CHECK(spx_ogs_gather((int)type, (int)op, k, NrowBlocksN, o_blockRowStartsN.ptr(), o_rowStartsN.ptr(),
                                o_colIdsN.ptr(), blockNnz, v, gv));
-  } else if (gatherSorted && NgsBlocks) {
+  } else if (trans != NoTrans && gatherSorted && srbRows) {
     const spx_srb_t d = gsSrb.Device();
     SPX_CHECK(spx_ogs_gather_srb((int)type, (int)op, k, &d, v, gv));
   } else {
@@ -128,11 +138,16 @@ void ogsOperator_t::ScatterRaw(Type type, Transpose trans, int k, const void *gv
       SPX_CHECK(spx_ogs_scatter_inverse((int)type, k, Ncols, (trans == Trans ? o_localRowN : o_localRowT).ptr(), gv, v));
     return;
   }
-  if (trans == Trans) {
+  const bool srbRows = NgsBlocks && gsSrb.rowsAscending;  // the layout numbers the rows
+  if (trans == Trans && scatterSorted && srbRows && (signedSrb || kind == Unsigned)) {
+    const spx_srb_t d = gsSrb.Device();
+    if (kind == Unsigned) SPX_CHECK(spx_ogs_scatter_srb((int)type, k, &d, gv, v));
+    else SPX_CHECK(spx_ogs_scatter_srb_signed((int)type, k, &d, gsSrb.DeviceMemberN(), NrowsN, gv, v));
+  } else if (trans == Trans) {
     if (NrowBlocksN)
       SPX_CHECK(spx_ogs_scatter((int)type, k, NrowBlocksN, o_blockRowStartsN.ptr(), o_rowStartsN.ptr(),
                                 o_colIdsN.ptr(), blockNnz, gv, v));
-  } else if (scatterSorted && NgsBlocks) {
+  } else if (trans != Trans && scatterSorted && srbRows) {
     const spx_srb_t d = gsSrb.Device();
     SPX_CHECK(spx_ogs_scatter_srb((int)type, k, &d, gv, v));
   } else {
@@ -154,6 +169,14 @@ void ogsOperator_t::GatherScatterRaw(Type type, Op op, Transpose trans, int k, v
       SPX_CHECK(spx_ogs_gather_scatter((int)type, (int)op, k, NgsBlocks, o_gsBlockRowStarts.ptr(), NrowsT,
                                        o_rowStartsT.ptr(), o_colIdsT.ptr(), NrowsT, o_rowStartsT.ptr(),
                                        o_colIdsT.ptr(), gsBlockNnz, v));
+    return;
+  }
+  if (NgsBlocks && gsSorted && (signedSrb || kind == Unsigned)) {
+    // Unsigned: the N-CSR is the T-CSR, so both transposes are the Sym form
+    const spx_srb_t d = gsSrb.Device();
+    if (kind == Unsigned) SPX_CHECK(spx_ogs_gather_scatter_srb((int)type, (int)op, k, &d, v));
+    else
+      SPX_CHECK(spx_ogs_gather_scatter_srb_signed((int)type, (int)op, (int)trans, k, &d, gsSrb.DeviceMemberN(), v));
     return;
   }
   if (!NgenBlocks) return;

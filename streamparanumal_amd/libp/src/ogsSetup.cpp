@@ -332,13 +332,18 @@ void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform
   kind = parts[0]->kind;
   unique = parts[0]->unique;
   gather_defined = true;
+  // Part i's T rows follow parts 0..i-1's; its N rows start at the same row
+  // (empty N rows fill the gap when a part has T rows without N members), so
+  // the union keeps N-CSR row r within T-CSR row r as one rank's operator
+  // does (ogsOperator.cpp:621-643) and the NoTrans / Trans forms act part by
+  // part.
   long long n = 0, rowsT = 0, rowsN = 0, nnzT = 0, nnzN = 0;
   for (const ogsBase_t *p : parts) {
     LIBP_ABORT("SetupBatch: a part is not set up", !p || !p->gatherLocal);
     LIBP_ABORT("SetupBatch: parts must share one Kind", p->kind != kind);
     n += p->N;
+    rowsN = rowsT + p->gatherLocal->NrowsN;
     rowsT += p->gatherLocal->NrowsT;
-    rowsN += p->gatherLocal->NrowsN;
     nnzT += p->gatherLocal->nnzT;
     nnzN += p->gatherLocal->nnzN;
     gather_defined = gather_defined && p->gather_defined;
@@ -346,7 +351,7 @@ void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform
   LIBP_ABORT("SetupBatch: " << n << " local nodes in all exceed dlong", n > std::numeric_limits<dlong>::max());
   N = (dlong)n;
   memory<dlong> rsT(rowsT + 1), ciT(nnzT > 0 ? nnzT : 1), rsN(rowsN + 1), ciN(nnzN > 0 ? nnzN : 1);
-  dlong nodeOff = 0, rowOffT = 0, rowOffN = 0, eOffT = 0, eOffN = 0;
+  dlong nodeOff = 0, rowOffT = 0, eOffT = 0, eOffN = 0;
   rsT[0] = 0;
   rsN[0] = 0;
   for (const ogsBase_t *p : parts) {
@@ -355,13 +360,16 @@ void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform
     for (dlong r = 0; r < op.NrowsT; ++r) rsT[rowOffT + r + 1] = eOffT + op.rowStartsT[r + 1];
 #pragma omp parallel for schedule(static)
     for (dlong e = 0; e < op.nnzT; ++e) ciT[eOffT + e] = nodeOff + op.colIdsT[e];
+    // N rows of this part, then empty ones up to the next part's first row
+    const dlong nN = std::min<dlong>(op.NrowsN, (dlong)(rowsN - rowOffT));
 #pragma omp parallel for schedule(static)
-    for (dlong r = 0; r < op.NrowsN; ++r) rsN[rowOffN + r + 1] = eOffN + op.rowStartsN[r + 1];
+    for (dlong r = 0; r < nN; ++r) rsN[rowOffT + r + 1] = eOffN + op.rowStartsN[r + 1];
+    const dlong gapEnd = std::min<dlong>(op.NrowsT, (dlong)(rowsN - rowOffT));
+    for (dlong r = nN; r < gapEnd; ++r) rsN[rowOffT + r + 1] = eOffN + op.nnzN;
 #pragma omp parallel for schedule(static)
     for (dlong e = 0; e < op.nnzN; ++e) ciN[eOffN + e] = nodeOff + op.colIdsN[e];
     nodeOff += p->N;
     rowOffT += op.NrowsT;
-    rowOffN += op.NrowsN;
     eOffT += op.nnzT;
     eOffN += op.nnzN;
   }

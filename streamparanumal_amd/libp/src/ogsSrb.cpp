@@ -20,7 +20,8 @@ int SrbChunksPerTile(int cap) {
 }
 
 SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowStarts, const memory<dlong> colIds,
-                                     const dlong Nblocks, const memory<dlong> blockStarts, const int cap) {
+                                     const dlong Nblocks, const memory<dlong> blockStarts, const int cap,
+                                     const uint8_t *memberFlag) {
   LIBP_ABORT("row block capacity must be in [1, " << maxBlockNnz << "]", cap < 1 || cap > maxBlockNnz);
   SortedRunBlocks S;
   S.nblocks = Nblocks;
@@ -35,10 +36,11 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
   }
   S.nchunks = Nblocks * S.cpt;
   S.chunks.malloc(S.nchunks > 0 ? S.nchunks : 1);
+  if (memberFlag) S.memberN.malloc(S.nchunks > 0 ? S.nchunks : 1);
   memory<dlong> sorted(S.nnz > 0 ? S.nnz : 1);   // local index at each position
   memory<dlong> prevPos(S.nnz > 0 ? S.nnz : 1);  // previous member's position, or -1
   std::vector<dlong> nruns(Nblocks, 0), nlinks(Nblocks, 0);
-  int duplicate = 0;
+  int duplicate = 0, unordered = 0;
 
   // pass 1: sort each tile by local index, link row members, masks and counts
 #pragma omp parallel
@@ -61,6 +63,15 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
         }
       }
       dlong links = 0;
+      // rows numbered by their leading positions: each row needs one, in row order
+      for (dlong r = blockStarts[b]; r < blockStarts[b + 1]; ++r) {
+        const bool empty = rowStarts[r + 1] == rowStarts[r];
+        const bool noPrev = r == blockStarts[b] || rowStarts[r] == rowStarts[r - 1];
+        if (empty || (!noPrev && posOfSlot[rowStarts[r] - e0] < posOfSlot[rowStarts[r - 1] - e0])) {
+#pragma omp atomic write
+          unordered = 1;
+        }
+      }
       for (dlong r = blockStarts[b]; r < blockStarts[b + 1]; ++r)
         for (dlong g = rowStarts[r]; g < rowStarts[r + 1]; ++g) {
           const dlong pos = posOfSlot[g - e0];
@@ -74,6 +85,7 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
         C.linkMask = 0;
         C.nvalid = (int32_t)std::max<dlong>(0, std::min<dlong>(64, n - 64 * k));
         C.chunkRow = blockStarts[b] + leading;
+        uint64_t member = 0;
         for (dlong l = 0; l < C.nvalid; ++l) {
           const dlong i = e0 + 64 * k + l;
           if (l == 0 || sorted[i] != sorted[i - 1] + 1) {
@@ -82,7 +94,9 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
           }
           if (prevPos[i] >= 0) C.linkMask |= 1ull << l;
           else ++leading;
+          if (memberFlag && memberFlag[sorted[i]]) member |= 1ull << l;
         }
+        if (memberFlag) S.memberN[(size_t)b * S.cpt + k] = member;
         if (C.nvalid == 0) {  // padding: decodes as the tile's first node and row (never stored)
           C.runMask = 1;
           C.chunkRow = blockStarts[b];
@@ -93,6 +107,7 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
     }
   }
   LIBP_ABORT("a local node appears twice in one row block", duplicate);
+  S.rowsAscending = !unordered;
   // pass 2: run and link tables
   std::vector<dlong> runStart(Nblocks + 1, 0), linkStart(Nblocks + 1, 0);
   for (dlong b = 0; b < Nblocks; ++b) {
@@ -126,6 +141,43 @@ void SortedRunBlocks::Upload(platform_t &platform) {
   o_chunks = platform.malloc<SrbChunkRecord>(chunks);
   o_runIds = platform.malloc<dlong>(runIds);
   o_links = platform.malloc<uint16_t>(links);
+  o_memberN = memberN.length() ? platform.malloc<uint64_t>(memberN) : deviceMemory<uint64_t>();
+}
+
+memory<uint8_t> SignedMemberFlags(const dlong Ncols, const dlong NrowsT, const memory<dlong> rowStartsT,
+                                  const memory<dlong> colIdsT, const dlong NrowsN, const memory<dlong> rowStartsN,
+                                  const memory<dlong> colIdsN) {
+  memory<uint8_t> flag(Ncols > 0 ? Ncols : 1);
+  if (NrowsN > NrowsT) return memory<uint8_t>();
+#pragma omp parallel for schedule(static)
+  for (dlong i = 0; i < Ncols; ++i) flag[i] = 0;
+  const dlong nnzN = NrowsN > 0 ? rowStartsN[NrowsN] : 0;
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (dlong e = 0; e < nnzN; ++e) {
+    const dlong c = colIdsN[e];
+    if (c < 0 || c >= Ncols) bad = 1;
+    else flag[c] = 1;
+  }
+  if (bad) return memory<uint8_t>();
+  // walk each T row's flagged members against its N row
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (dlong r = 0; r < NrowsT; ++r) {
+    dlong n = r < NrowsN ? rowStartsN[r] : 0;
+    const dlong nEnd = r < NrowsN ? rowStartsN[r + 1] : 0;
+    for (dlong g = rowStartsT[r]; g < rowStartsT[r + 1]; ++g) {
+      const dlong c = colIdsT[g];
+      if (!flag[c]) continue;
+      if (n >= nEnd || colIdsN[n] != c) {
+        bad = 1;
+        break;
+      }
+      ++n;
+    }
+    if (n != nEnd) bad = 1;
+  }
+  if (bad) return memory<uint8_t>();
+  return flag;
 }
 
 spx_srb_t SortedRunBlocks::Device() const {

@@ -98,14 +98,24 @@ SRB_CHUNK = np.dtype([("runMask", "<u8"), ("linkMask", "<u8"), ("runBase", "<i4"
                       ("chunkRow", "<i4"), ("nvalid", "<i4")])
 
 
-def srb_build(row_starts, col_ids, block_starts, cap):
-    """Sorted-run tiles (spx_srb_t) of a CSR and its row blocks, as host arrays."""
+def srb_build(row_starts, col_ids, block_starts, cap, n_csr=None, ncols=None):
+    """Sorted-run tiles (spx_srb_t) of a CSR and its row blocks, as host arrays.  n_csr = (rowStartsN,
+    colIdsN) of a Signed operator whose T-CSR this is (ncols local nodes) adds its N-member masks
+    ("memberN")."""
     rs = np.ascontiguousarray(row_starts, dtype=np.int32)
     ci = np.ascontiguousarray(col_ids, dtype=np.int32)
     bs = np.ascontiguousarray(block_starts, dtype=np.int32)
     h = C.c_void_p()
-    checkh(libspxhost().spxh_srb_build(rs.size - 1, rs.ctypes.data, ci.ctypes.data if ci.size else None,
-                                       bs.size - 1, bs.ctypes.data, cap, C.byref(h)), "spxh_srb_build")
+    if n_csr is None:
+        checkh(libspxhost().spxh_srb_build(rs.size - 1, rs.ctypes.data, ci.ctypes.data if ci.size else None,
+                                           bs.size - 1, bs.ctypes.data, cap, C.byref(h)), "spxh_srb_build")
+    else:
+        rsn = np.ascontiguousarray(n_csr[0], dtype=np.int32)
+        cin = np.ascontiguousarray(n_csr[1], dtype=np.int32)
+        checkh(libspxhost().spxh_srb_build_signed(rs.size - 1, rs.ctypes.data, ci.ctypes.data if ci.size else None,
+                                                  bs.size - 1, bs.ctypes.data, cap, ncols, rsn.size - 1,
+                                                  rsn.ctypes.data, cin.ctypes.data if cin.size else None,
+                                                  C.byref(h)), "spxh_srb_build_signed")
     try:
         sz = np.zeros(6, dtype=np.int64)
         checkh(libspxhost().spxh_srb_sizes(h, sz.ctypes.data), "spxh_srb_sizes")
@@ -115,6 +125,13 @@ def srb_build(row_starts, col_ids, block_starts, cap):
         checkh(libspxhost().spxh_srb_copy(h, *(out[k].ctypes.data for k in ("chunks", "runIds", "links",
                                                                                 "entryStarts"))), "spxh_srb_copy")
         out.update(nblocks=nb, nnz=nnz, nchunks=nch, nruns=nruns, nlinks=nlinks, chunks_per_tile=cpt)
+        asc = C.c_int32()
+        checkh(libspxhost().spxh_srb_rows_ascending(h, C.byref(asc)), "spxh_srb_rows_ascending")
+        out["rows_ascending"] = bool(asc.value)  # the gather and scatter need it
+        if n_csr is not None:
+            out["memberN"] = np.zeros(max(nch, 1), np.uint64)
+            checkh(libspxhost().spxh_srb_copy_members(h, out["memberN"].ctypes.data), "spxh_srb_copy_members")
+            out["nrowsN"] = np.asarray(n_csr[0]).size - 1
         return out
     finally:
         libspxhost().spxh_srb_destroy(h)
@@ -128,8 +145,28 @@ class SrbDevice:
         self.arrays = {"chunks": DeviceArray.from_numpy(srb["chunks"].view(np.uint8)),
                        "runIds": DeviceArray.from_numpy(srb["runIds"]),
                        "links": DeviceArray.from_numpy(srb["links"])}
+        if "memberN" in srb:
+            self.arrays["memberN"] = DeviceArray.from_numpy(srb["memberN"])
+            self.nrowsN = srb["nrowsN"]
         self.desc = _lib.Srb(srb["nblocks"], srb["chunks_per_tile"], xcd_remap, tile_lanes,
                              *(self.arrays[k].ptr for k in ("chunks", "runIds", "links")))
+
+    def gather_scatter_signed(self, q, k=1, op=_lib.ADD, trans=_lib.NOTRANS):
+        check(libspx().spx_ogs_gather_scatter_srb_signed(ogs_type(q.dtype), op, trans, k, C.byref(self.desc),
+                                                         self.arrays["memberN"].ptr, q.ptr),
+              "spx_ogs_gather_scatter_srb_signed")
+
+    def gather_signed(self, gq, q, k=1, op=_lib.ADD):
+        """The N-CSR (NoTrans) gather."""
+        check(libspx().spx_ogs_gather_srb_signed(ogs_type(q.dtype), op, k, C.byref(self.desc),
+                                                 self.arrays["memberN"].ptr, self.nrowsN, q.ptr, gq.ptr),
+              "spx_ogs_gather_srb_signed")
+
+    def scatter_signed(self, q, gq, k=1):
+        """The N-CSR (Trans) scatter."""
+        check(libspx().spx_ogs_scatter_srb_signed(ogs_type(q.dtype), k, C.byref(self.desc),
+                                                  self.arrays["memberN"].ptr, self.nrowsN, gq.ptr, q.ptr),
+              "spx_ogs_scatter_srb_signed")
 
     def gather_scatter(self, q, k=1, op=_lib.ADD):
         check(libspx().spx_ogs_gather_scatter_srb(ogs_type(q.dtype), op, k, C.byref(self.desc), q.ptr),

@@ -268,6 +268,151 @@ def test_srb_direct_random_ids(dev, oracle, seed, cap):
     assert same(dq2.to_numpy(), oracle.csr_gs(rs, ci, q))
 
 
+def op_values(n, dtype, seed, k, op):
+    """values(); for MUL near 1, so products over long rows stay finite (and integers do not
+    overflow)."""
+    q = values(n, dtype, seed, k)
+    if op == spx.MUL:
+        q = (np.ones_like(q) + (q % 3 if np.dtype(dtype).kind == "i" else q * 0.01)).astype(dtype)
+    return q
+
+
+def signed_ids(n, seed, oracle, ordered=False):
+    """Random ids with repeats, negatives and zeros: without unique flagging some groups have several
+    positive members and some none.  ordered: the groups with no positive member lie in the second half
+    only and every other group's first member is positive, so the rows (positive groups first) ascend
+    by first member."""
+    ids = (oracle.rand_int(n, 300 + seed) % (n // 3) + 1).astype(np.int64)
+    r = oracle.rand_int(n, 400 + seed)
+    if ordered:
+        h = n // 2
+        ids[h:] = -ids[h:]                                         # members of earlier groups, or
+        fresh = np.nonzero(r[h:] % 3 == 0)[0] + h
+        ids[fresh] = -(n + r[fresh] % (n // 5))                    # groups with no positive member
+        ids[r % 11 == 0] = 0
+        live = np.nonzero(ids[:h])[0]
+        _, first = np.unique(ids[live], return_index=True)        # each group's first member stays positive
+        neg = np.zeros(h, bool)
+        neg[live] = r[live] % 4 == 0
+        neg[live[first]] = False
+        ids[:h][neg] *= -1
+        return ids
+    ids[r % 4 == 0] *= -1
+    ids[r % 11 == 0] = 0
+    return ids
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("seed,unique,ordered,cap", [(0, False, False, 256), (1, False, False, 4096),
+                                                     (2, True, False, 700), (3, False, True, 256),
+                                                     (4, False, True, 2048)])
+def test_srb_signed_direct(dev, oracle, dtype, op, seed, unique, ordered, cap):
+    """The N-CSR forms through the C ABI on a layout with N-member masks: NoTrans and Trans
+    gather+scatter, the N-CSR gather and the N-CSR scatter, bit-exact with the oracle's CSR loops."""
+    say("srb signed direct", np.dtype(dtype).name, op, seed, unique, ordered, cap)
+    n = 20000
+    ids = signed_ids(n, seed, oracle, ordered)
+    o = oracle.Ogs(ids, kind=spx.SIGNED, unique=unique)
+    _, c = ogs.build_csr(ids, spx.SIGNED, unique)
+    rs, ci = c["rowStartsT"], c["colIdsT"]
+    bs = ogs.block_rows(rs, cap)
+    lay = ogs.srb_build(rs, ci, bs, cap, n_csr=(c["rowStartsN"], c["colIdsN"]), ncols=n)
+    d = ogs.SrbDevice(lay)
+    # without unique flagging, groups with no positive member follow the others: the rows of the tile
+    # where they start do not ascend, and only the gather+scatter forms (no row numbers) apply
+    assert lay["rows_ascending"] == (unique or ordered)
+    if ordered:   # N rows of several members, and T rows with none
+        assert c["NrowsN"] < c["NrowsT"] and np.diff(c["rowStartsN"]).max() > 1
+    for k in (1, 3):
+        q = op_values(n, dtype, seed + 10 * k, k, op)
+        for trans in (spx.NOTRANS, spx.TRANS, spx.SYM):
+            dq = DeviceArray.from_numpy(q)
+            d.gather_scatter_signed(dq, k, op, trans)
+            device.check_kernel_errors()
+            assert same(dq.to_numpy(), o.gather_scatter(q, k, op, trans)), (k, trans)
+        if not lay["rows_ascending"]:
+            continue
+        dg = DeviceArray(c["NrowsN"] * k, np.dtype(dtype))
+        d.gather_signed(dg, DeviceArray.from_numpy(q), k, op)
+        device.check_kernel_errors()
+        assert same(dg.to_numpy(), o.gather(q, k, op, spx.NOTRANS)), k
+        gsrc = values(c["NrowsN"], dtype, seed + 20 * k, k)
+        base = values(n, dtype, seed + 30 * k, k)
+        dv = DeviceArray.from_numpy(base)
+        d.scatter_signed(dv, DeviceArray.from_numpy(gsrc), k)
+        device.check_kernel_errors()
+        assert same(dv.to_numpy(), o.scatter(gsrc, base, k, spx.TRANS)), k
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("trans", [spx.NOTRANS, spx.TRANS])
+@pytest.mark.parametrize("unique", [False, True])
+def test_signed_ids_non_sym_through_ogs(dev, oracle, dtype, trans, unique):
+    """ogs_t's NoTrans / Trans gather+scatter (and, where the gather is defined, the N-CSR gather and
+    scatter) on signed random ids, on the sorted-run layout and on the CSR row blocks."""
+    say("signed ogs", np.dtype(dtype).name, trans, unique)
+    n = 30000
+    ids = signed_ids(n, 7, oracle)
+    g, o = ogs.Ogs(ids, kind=spx.SIGNED, unique=unique), oracle.Ogs(ids, kind=spx.SIGNED, unique=unique)
+    assert g.gather_defined == unique
+    for op in OPS:
+        for k in (1, 2):
+            q = op_values(n, dtype, 5 + k, k, op)
+            want = o.gather_scatter(q, k, op, trans)
+            for srb in (True, False):
+                g.set_gs_format(srb)
+                dq = DeviceArray.from_numpy(q)
+                g.gather_scatter(dq, k, op, trans)
+                device.check_kernel_errors()
+                assert same(dq.to_numpy(), want), (op, k, srb)
+            g.set_gs_format(True)
+            if not unique:
+                continue
+            gtrans, strans = (spx.NOTRANS, spx.TRANS)          # the N-CSR forms
+            for srb in (True, False):
+                g.set_gather_format(srb, srb)
+                dg = DeviceArray(g.NlocalP * k, np.dtype(dtype))
+                g.gather(dg, DeviceArray.from_numpy(q), k, op, gtrans)
+                device.check_kernel_errors()
+                assert same(dg.to_numpy(), o.gather(q, k, op, gtrans)), (op, k, srb)
+                base = values(n, dtype, 9, k)
+                dv = DeviceArray.from_numpy(base)
+                g.scatter(dv, dg, k, strans)
+                device.check_kernel_errors()
+                assert same(dv.to_numpy(), o.scatter(o.gather(q, k, op, gtrans), base, k, strans)), (op, k, srb)
+            g.set_gather_format(True, True)
+
+
+def test_batch_non_sym_acts_part_by_part(dev, oracle):
+    """A batch whose first part has groups without positive members: the union keeps each part's
+    N-CSR rows at its T-CSR rows (empty N rows fill the gap), so the NoTrans / Trans forms act part by
+    part, on the sorted-run layout and on the CSR row blocks."""
+    parts_ids = [signed_ids(5000, 11, oracle), signed_ids(4000, 12, oracle)]
+    parts = [ogs.Ogs(i, kind=spx.SIGNED, unique=False) for i in parts_ids]
+    assert parts[0].NlocalP < parts[0].NlocalT
+    b = ogs.Ogs.batch(parts)
+    say("batch non-sym", b.N)
+    assert b.NlocalT == parts[0].NlocalT + parts[1].NlocalT
+    assert b.NlocalP == parts[0].NlocalT + parts[1].NlocalP
+    rsN, ciN, _ = b.csr("N")
+    rsN0, ciN0, _ = parts[0].csr("N")
+    assert np.array_equal(rsN[:parts[0].NlocalP + 1], rsN0)
+    assert np.all(rsN[parts[0].NlocalP:parts[0].NlocalT + 1] == rsN0[-1])      # the gap rows are empty
+    q = values(b.N, np.float64, 3)
+    offs = (0, parts_ids[0].size)
+    for trans in (spx.NOTRANS, spx.TRANS):
+        want = np.concatenate([oracle.Ogs(i, kind=spx.SIGNED, unique=False).gather_scatter(
+            q[off:off + i.size], 1, spx.ADD, trans) for i, off in zip(parts_ids, offs)])
+        for srb in (True, False):
+            b.set_gs_format(srb)
+            dq = DeviceArray.from_numpy(q)
+            b.gather_scatter(dq, 1, spx.ADD, trans)
+            device.check_kernel_errors()
+            assert same(dq.to_numpy(), want), (trans, srb)
+    b.set_gs_format(True)
+
+
 def test_reference_bisection_blocks(setups, oracle):
     """The kernel run on the reference's own row blocks (<=512 entries, ogsOperator.cpp:522-594)."""
     mesh = (8, 8, 8, 7)

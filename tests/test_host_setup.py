@@ -123,6 +123,7 @@ def test_srb_layout_matches_csr(mesh, cap):
     assert s["nblocks"] == bs.size - 1 and s["nnz"] == ci.size and s["nchunks"] == s["nblocks"] * cpt
     assert np.array_equal(s["entryStarts"], rs[bs])
     assert s["nlinks"] == ci.size - (rs.size - 1)            # every member but the first of each row
+    assert s["rows_ascending"]                                # rows numbered by their leading positions
     for b, ent in enumerate(decode_srb(s)):
         e0, e1 = rs[bs[b]], rs[bs[b + 1]]
         n = e1 - e0
@@ -146,6 +147,67 @@ def test_srb_layout_matches_csr(mesh, cap):
     for rec in s["chunks"][: s["nchunks"]]:
         assert int(rec["runMask"]) & 1                       # every record decodes (padding included)
         assert 0 <= rec["runBase"] < max(s["nruns"], 1)
+
+
+def signed_ids(n, seed):
+    """Random ids with repeats, negatives (groups with no positive member among them) and zeros."""
+    ids = (o.rand_int(n, 300 + seed) % (n // 3) + 1).astype(np.int64)
+    r = o.rand_int(n, 400 + seed)
+    ids[r % 4 == 0] *= -1
+    ids[r % 11 == 0] = 0
+    return ids
+
+
+@pytest.mark.parametrize("seed,unique,cap", [(0, False, 64), (1, False, 700), (2, True, 256), (3, True, 4096)])
+def test_srb_member_masks(seed, unique, cap):
+    """Bit l of record k's mask is set exactly when position l holds a member of the N-CSR."""
+    n = 3000
+    _, c = ogs.build_csr(signed_ids(n, seed), spx.SIGNED, unique)
+    rs, ci = c["rowStartsT"], c["colIdsT"]
+    if unique:                                             # one positive member per group
+        assert c["NrowsN"] == c["NrowsT"] and np.all(np.diff(c["rowStartsN"]) == 1)
+    else:                                                  # groups with none, and with several
+        assert c["NrowsN"] < c["NrowsT"] and np.diff(c["rowStartsN"]).max() > 1
+    bs = ogs.block_rows(rs, cap)
+    s = ogs.srb_build(rs, ci, bs, cap, n_csr=(c["rowStartsN"], c["colIdsN"]), ncols=n)
+    assert s["nrowsN"] == c["NrowsN"]
+    # without unique flagging the groups with no positive member follow the others (ogsSetup.cpp:517-546),
+    # so the tile where they start holds rows out of first-member order: no row numbers on this layout
+    first = ci[rs[:-1]]
+    assert s["rows_ascending"] == bool(np.all(np.diff(first) > 0)) == unique
+    members = set(int(x) for x in c["colIdsN"])
+    cpt = s["chunks_per_tile"]
+    for b, ent in enumerate(decode_srb(s)):
+        for k in range(cpt):
+            m = int(s["memberN"][b * cpt + k])
+            chunk = ent[64 * k: 64 * (k + 1)]
+            assert m >> len(chunk) == 0                    # no bits past the entries
+            assert [bool(m >> l & 1) for l in range(len(chunk))] == [e[0] in members for e in chunk]
+    plain = ogs.srb_build(rs, ci, bs, cap)
+    assert np.array_equal(plain["chunks"], s["chunks"]) and np.array_equal(plain["links"], s["links"])
+
+
+def test_srb_signed_rejects_misaligned_n_csr():
+    """N-CSR rows that are not the members of the same T-CSR rows are refused (the kernels would
+    fold the wrong rows)."""
+    rs = np.array([0, 2, 4], np.int32)
+    ci = np.array([0, 1, 2, 3], np.int32)
+    bs = np.array([0, 2], np.int32)
+    ok = ogs.srb_build(rs, ci, bs, 64, n_csr=(np.array([0, 1, 3], np.int32), np.array([1, 2, 3], np.int32)),
+                       ncols=4)
+    assert int(ok["memberN"][0]) == 0b1110
+    bad = [((0, 1, 2), (2, 0)),        # rows swapped
+           ((0, 1), (2,)),             # row 0's N member lies in T row 1
+           ((0, 2), (1, 0)),           # members out of order
+           ((0, 0, 1), (3,)),          # an empty N row, then a member of T row 1 in N row 1 -- allowed
+           ((0, 1, 1, 1), (0,))]       # more N rows than T rows
+    for i, (rsn, cin) in enumerate(bad):
+        args = (np.array(rsn, np.int32), np.array(cin, np.int32))
+        if i == 3:
+            assert int(ogs.srb_build(rs, ci, bs, 64, n_csr=args, ncols=4)["memberN"][0]) == 0b1000
+            continue
+        with pytest.raises(SpxError):
+            ogs.srb_build(rs, ci, bs, 64, n_csr=args, ncols=4)
 
 
 def test_srb_rejects_bad_blocks():
