@@ -75,6 +75,14 @@ int spxh_ogs_set_scatter_path(spxh_ogs_t ogs, int by_rows);
    default), 0 = the T-CSR row blocks; results are identical.  xcd_remap is the
    sorted-run launch hint (-1 library default, 0 hardware order, 1 remap). */
 int spxh_ogs_set_gs_format(spxh_ogs_t ogs, int sorted, int xcd_remap);
+/* Row order of the sorted-run gather+scatter tiles (no reference counterpart):
+   the T-CSR rows sorted by keys[r] (NlocalT keys; ties in row order), or the
+   CSR's own order when keys is NULL.  The gather+scatter needs no row
+   numbers, so results are identical; rows whose members share elements
+   should share tiles.  A hex box's setup (spxh_ogs_setup_hex, mesh_t) orders
+   its grid points in small 3-D blocks by itself; the gather and scatter keep
+   the CSR's order. */
+int spxh_ogs_set_gs_row_keys(spxh_ogs_t ogs, const int64_t *keys);
 /* layouts of the T-CSR gather (Trans) and scatter (NoTrans): 1 = the
    sorted-run blocks of the gather+scatter, 0 = T-CSR row blocks, -1 = keep;
    both sorted by default.  Results are identical. */

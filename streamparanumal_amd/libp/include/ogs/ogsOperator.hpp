@@ -116,9 +116,14 @@ class ogsOperator_t {
   memory<dlong> gsBlockRowStarts;
   deviceMemory<dlong> o_gsBlockRowStarts;
 
-  // The Sym gather+scatter runs on the sorted-run layout of the gs row
-  // blocks (gsSorted) or directly on the CSR.
-  SortedRunBlocks gsSrb;
+  // The gather+scatter runs on a sorted-run layout (gsSorted) or directly on
+  // the CSR.  It needs no row numbers, so its tiles may take the rows in
+  // another order (gsRowOrder, empty: the CSR's), e.g. a hex box's grid
+  // points in small 3-D blocks, whose members then share elements and cache
+  // lines; the gather and scatter keep the CSR's row order (rowSrb, the same
+  // layout as gsSrb when gsRowOrder is empty).
+  SortedRunBlocks gsSrb, rowSrb;
+  memory<dlong> gsRowOrder;
   bool gsSorted = true;
   // The T-CSR gather (Trans) and scatter (NoTrans) on the same layout or on
   // the T-CSR row blocks of blockNnz entries.  Measured on MI355X at 64^3 hex
@@ -159,6 +164,11 @@ class ogsOperator_t {
   // scatter blocks, and gather+scatter blocks.
   void SetBlockNnz(int nnz);
   void SetGsBlockNnz(int nnz);
+  // Rows of the gather+scatter tiles in this order (a permutation of the
+  // T-CSR rows; empty: the CSR's); results do not change.
+  void SetGsRowOrder(const memory<dlong> &order);
+  // gsSrb from rowSrb and gsRowOrder, with these launch hints
+  void BuildGsSrb(int xcdRemap, int tileLanes);
 
   template <typename T> void Gather(deviceMemory<T> gv, const deviceMemory<T> v, const int k, const Op op, const Transpose trans);
   template <typename T> void Scatter(deviceMemory<T> v, const deviceMemory<T> gv, const int k, const Transpose trans);
@@ -194,6 +204,16 @@ LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const 
 // BuildLocalCSR would leave them.
 LocalCSR BuildHexBoxCSR(const dlong nx, const dlong ny, const dlong nz, const int p, const Kind kind,
                         const bool unique, const int rank, hlong *idsOut);
+
+// Row order of a hex box's gather+scatter tiles: its grid points (i fastest)
+// in blocks of bx x by x bz points, blocks in lexicographic order, rows in CSR
+// order within a block.  rowGrid[r] is row r's lexicographic grid index on an
+// Mx x My x Mz lattice.
+memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid, const hlong Mx, const hlong My,
+                                const hlong Mz, const int bx, const int by, const int bz);
+// Block shape measured best on MI355X for a hex box of degree p (64^3 p=7 and
+// ~1e8-node boxes at p = 1..15).
+void HexGsBlockShape(const int p, int &bx, int &by, int &bz);
 
 // Greedy row blocking: consecutive rows, at most `cap` entries per block.
 // Aborts when one row alone exceeds cap (ogsOperator.cpp:535-536).

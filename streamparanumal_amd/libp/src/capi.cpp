@@ -195,6 +195,25 @@ int spxh_ogs_set_gs_format(spxh_ogs_t h, int sorted, int xcd_remap) {
     LIBP_ABORT("ogs not set up", !op);
     op->gsSorted = sorted != 0;
     op->gsSrb.xcdRemap = xcd_remap;
+    op->rowSrb.xcdRemap = xcd_remap;
+  });
+}
+
+int spxh_ogs_set_gs_row_keys(spxh_ogs_t h, const int64_t *keys) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_row_keys: null");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    memory<dlong> order;
+    if (keys) {
+      const dlong n = op->NrowsT;
+      std::vector<std::pair<int64_t, dlong>> kr(n);
+      for (dlong r = 0; r < n; ++r) kr[r] = {keys[r], r};
+      std::sort(kr.begin(), kr.end());
+      order.malloc(n);
+      for (dlong r = 0; r < n; ++r) order[r] = kr[r].second;
+    }
+    op->SetGsRowOrder(order);
   });
 }
 

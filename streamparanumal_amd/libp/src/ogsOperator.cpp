@@ -1,6 +1,10 @@
 // ogsOperator_t on the device (libs/ogs/ogsOperator.cpp:176-204, 295-322,
 // 442-481, 597-665) over the libspx ogs kernels.
 #include <algorithm>
+#include <climits>
+#include <vector>
+
+#include <omp.h>
 
 #include "ogs.hpp"
 
@@ -78,11 +82,110 @@ void ogsOperator_t::SetGsBlockNnz(int nnz) {
   blockRowsGreedy(NrowsT, rowStartsT, gsBlockNnz, NgsBlocks, gsBlockRowStarts);
   o_gsBlockRowStarts = platform.malloc<dlong>(NgsBlocks + 1, gsBlockRowStarts);
   const int xcd = gsSrb.xcdRemap, lanes = gsSrb.tileLanes;
-  gsSrb = BuildSortedRunBlocks(NrowsT, rowStartsT, colIdsT, NgsBlocks, gsBlockRowStarts, gsBlockNnz,
-                               signedSrb ? memberFlagN.ptr() : nullptr);
+  const int rxcd = rowSrb.xcdRemap, rlanes = rowSrb.tileLanes;
+  const uint8_t *flags = signedSrb ? memberFlagN.ptr() : nullptr;
+  rowSrb = BuildSortedRunBlocks(NrowsT, rowStartsT, colIdsT, NgsBlocks, gsBlockRowStarts, gsBlockNnz, flags);
+  rowSrb.xcdRemap = rxcd;
+  rowSrb.tileLanes = rlanes;
+  rowSrb.Upload(platform);
+  BuildGsSrb(xcd, lanes);
+}
+
+void ogsOperator_t::BuildGsSrb(int xcd, int lanes) {
+  const uint8_t *flags = signedSrb ? memberFlagN.ptr() : nullptr;
+  if (!gsRowOrder.length()) {
+    gsSrb = rowSrb;
+  } else {
+    // the T-CSR with its rows in gsRowOrder, cut into greedy blocks
+    memory<dlong> rs(NrowsT + 1), ci(nnzT > 0 ? nnzT : 1);
+    rs[0] = 0;
+    for (dlong i = 0; i < NrowsT; ++i) {
+      const dlong r = gsRowOrder[i];
+      rs[i + 1] = rs[i] + (rowStartsT[r + 1] - rowStartsT[r]);
+    }
+#pragma omp parallel for schedule(static)
+    for (dlong i = 0; i < NrowsT; ++i) {
+      const dlong r = gsRowOrder[i];
+      std::copy(colIdsT.ptr() + rowStartsT[r], colIdsT.ptr() + rowStartsT[r + 1], ci.ptr() + rs[i]);
+    }
+    dlong nb = 0;
+    memory<dlong> bs;
+    blockRowsGreedy(NrowsT, rs, gsBlockNnz, nb, bs);
+    gsSrb = BuildSortedRunBlocks(NrowsT, rs, ci, nb, bs, gsBlockNnz, flags);
+    gsSrb.Upload(platform);
+  }
   gsSrb.xcdRemap = xcd;
   gsSrb.tileLanes = lanes;
-  gsSrb.Upload(platform);
+}
+
+void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order) {
+  if (order.length()) {
+    LIBP_ABORT("gather+scatter row order must list the " << NrowsT << " rows", (dlong)order.length() != NrowsT);
+    std::vector<char> seen(NrowsT, 0);
+    for (dlong i = 0; i < NrowsT; ++i) {
+      const dlong r = order[i];
+      LIBP_ABORT("gather+scatter row order is not a permutation of the rows", r < 0 || r >= NrowsT || seen[r]);
+      seen[r] = 1;
+    }
+  }
+  gsRowOrder = order;
+  BuildGsSrb(gsSrb.xcdRemap, gsSrb.tileLanes);
+}
+
+memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid, const hlong Mx, const hlong My,
+                                const hlong Mz, const int bx, const int by, const int bz) {
+  LIBP_ABORT("lattice blocks need positive sizes", Mx < 1 || My < 1 || Mz < 1 || bx < 1 || by < 1 || bz < 1);
+  const hlong nbx = (Mx + bx - 1) / bx, nby = (My + by - 1) / by, nbz = (Mz + bz - 1) / bz;
+  const hlong nkeys = nbx * nby * nbz;
+  LIBP_ABORT("too many lattice blocks (" << nkeys << ")", nkeys > (hlong)INT32_MAX);
+  memory<dlong> order(Nrows > 0 ? Nrows : 1);
+  if (Nrows == 0) return order;
+  std::vector<int32_t> key(Nrows);
+#pragma omp parallel for schedule(static)
+  for (dlong r = 0; r < Nrows; ++r) {
+    const hlong g = rowGrid[r];
+    const hlong i = g % Mx, j = (g / Mx) % My, k = g / (Mx * My);
+    key[r] = (int32_t)(((k / bz) * nby + j / by) * nbx + i / bx);
+  }
+  // stable counting sort by block: each thread counts its contiguous share of
+  // the rows, the counts are scanned block-major then thread-major, and each
+  // thread places its rows in row order
+  const int T = std::max(1, std::min(omp_get_max_threads(), (int)((Nrows + 65535) / 65536)));
+  std::vector<int64_t> count((size_t)T * nkeys, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    const dlong r0 = (dlong)((int64_t)Nrows * t / T), r1 = (dlong)((int64_t)Nrows * (t + 1) / T);
+    int64_t *c = count.data() + (size_t)t * nkeys;
+    for (dlong r = r0; r < r1; ++r) ++c[key[r]];
+#pragma omp barrier
+#pragma omp single
+    {
+      int64_t acc = 0;
+      for (hlong b = 0; b < nkeys; ++b)
+        for (int u = 0; u < T; ++u) {
+          int64_t &x = count[(size_t)u * nkeys + b];
+          const int64_t n = x;
+          x = acc;
+          acc += n;
+        }
+    }
+    for (dlong r = r0; r < r1; ++r) order[c[key[r]]++] = r;
+  }
+  return order;
+}
+
+void HexGsBlockShape(const int p, int &bx, int &by, int &bz) {
+  // 64^3 p=7 and ~1e8-node boxes (us, default order -> blocks): p=1 640 -> 436
+  // with 8x4x4, p=2 601 -> 447 with 8x8x4, p=3 387 -> 361 and p=7 470 -> 461,
+  // p=15 357 -> 312 with 16x8x2
+  if (p <= 1) {
+    bx = 8; by = 4; bz = 4;
+  } else if (p == 2) {
+    bx = 8; by = 8; bz = 4;
+  } else {
+    bx = 16; by = 8; bz = 2;
+  }
 }
 
 void ogsOperator_t::SetBlockNnz(int nnz) {
@@ -112,17 +215,17 @@ void ogsOperator_t::SetBlockNnz(int nnz) {
 void ogsOperator_t::Free() { *this = ogsOperator_t(); }
 
 void ogsOperator_t::GatherRaw(Type type, Op op, Transpose trans, int k, const void *v, void *gv) {
-  const bool srbRows = NgsBlocks && gsSrb.rowsAscending;  // the layout numbers the rows
+  const bool srbRows = NgsBlocks && rowSrb.rowsAscending;  // the layout numbers the rows
   if (trans == NoTrans && gatherSorted && srbRows && (signedSrb || kind == Unsigned)) {
-    const spx_srb_t d = gsSrb.Device();
+    const spx_srb_t d = rowSrb.Device();
     if (kind == Unsigned) SPX_CHECK(spx_ogs_gather_srb((int)type, (int)op, k, &d, v, gv));
-    else SPX_CHECK(spx_ogs_gather_srb_signed((int)type, (int)op, k, &d, gsSrb.DeviceMemberN(), NrowsN, v, gv));
+    else SPX_CHECK(spx_ogs_gather_srb_signed((int)type, (int)op, k, &d, rowSrb.DeviceMemberN(), NrowsN, v, gv));
   } else if (trans == NoTrans) {
     if (NrowBlocksN)
       SPX_CHECK(spx_ogs_gather((int)type, (int)op, k, NrowBlocksN, o_blockRowStartsN.ptr(), o_rowStartsN.ptr(),
                                o_colIdsN.ptr(), blockNnz, v, gv));
   } else if (trans != NoTrans && gatherSorted && srbRows) {
-    const spx_srb_t d = gsSrb.Device();
+    const spx_srb_t d = rowSrb.Device();
     SPX_CHECK(spx_ogs_gather_srb((int)type, (int)op, k, &d, v, gv));
   } else {
     if (NrowBlocksT)
@@ -138,17 +241,17 @@ void ogsOperator_t::ScatterRaw(Type type, Transpose trans, int k, const void *gv
       SPX_CHECK(spx_ogs_scatter_inverse((int)type, k, Ncols, (trans == Trans ? o_localRowN : o_localRowT).ptr(), gv, v));
     return;
   }
-  const bool srbRows = NgsBlocks && gsSrb.rowsAscending;  // the layout numbers the rows
+  const bool srbRows = NgsBlocks && rowSrb.rowsAscending;  // the layout numbers the rows
   if (trans == Trans && scatterSorted && srbRows && (signedSrb || kind == Unsigned)) {
-    const spx_srb_t d = gsSrb.Device();
+    const spx_srb_t d = rowSrb.Device();
     if (kind == Unsigned) SPX_CHECK(spx_ogs_scatter_srb((int)type, k, &d, gv, v));
-    else SPX_CHECK(spx_ogs_scatter_srb_signed((int)type, k, &d, gsSrb.DeviceMemberN(), NrowsN, gv, v));
+    else SPX_CHECK(spx_ogs_scatter_srb_signed((int)type, k, &d, rowSrb.DeviceMemberN(), NrowsN, gv, v));
   } else if (trans == Trans) {
     if (NrowBlocksN)
       SPX_CHECK(spx_ogs_scatter((int)type, k, NrowBlocksN, o_blockRowStartsN.ptr(), o_rowStartsN.ptr(),
                                 o_colIdsN.ptr(), blockNnz, gv, v));
   } else if (trans != Trans && scatterSorted && srbRows) {
-    const spx_srb_t d = gsSrb.Device();
+    const spx_srb_t d = rowSrb.Device();
     SPX_CHECK(spx_ogs_scatter_srb((int)type, k, &d, gv, v));
   } else {
     if (NrowBlocksT)

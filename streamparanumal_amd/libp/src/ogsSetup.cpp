@@ -304,6 +304,24 @@ void ogsBase_t::SetupHexBox(const dlong nx, const dlong ny, const dlong nz, cons
   LocalCSR csr = BuildHexBoxCSR(nx, ny, nz, p, kind, unique, comm.rank(), idsOut);
   N = csr.nnzT;
   SetupFromCSR(csr);
+  // gather+scatter tiles over small 3-D blocks of grid points: a row's grid
+  // point from its first member (element e, node a + b*Nq + c*Nq^2, elements
+  // x fastest, meshSetupBoxHex3D.cpp:87-130)
+  {
+    const ogsOperator_t &op = *gatherLocal;
+    const hlong Nq = p + 1, Np = Nq * Nq * Nq;
+    const hlong Mx = (hlong)nx * p + 1, My = (hlong)ny * p + 1, Mz = (hlong)nz * p + 1;
+    memory<hlong> grid(op.NrowsT > 0 ? op.NrowsT : 1);
+#pragma omp parallel for schedule(static)
+    for (dlong r = 0; r < op.NrowsT; ++r) {
+      const hlong l = op.colIdsT[op.rowStartsT[r]], e = l / Np, n = l % Np;
+      const hlong ex = e % nx, ey = (e / nx) % ny, ez = e / ((hlong)nx * ny);
+      grid[r] = (ex * p + n % Nq) + (ey * p + (n / Nq) % Nq) * Mx + (ez * p + n / (Nq * Nq)) * Mx * My;
+    }
+    int bx, by, bz;
+    HexGsBlockShape(p, bx, by, bz);
+    gatherLocal->SetGsRowOrder(LatticeBlockOrder(op.NrowsT, grid, Mx, My, Mz, bx, by, bz));
+  }
   setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -381,6 +399,21 @@ void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform
   NgatherGlobal = Ngather;
   gatherLocal = std::make_shared<ogsOperator_t>(platform, kind == Unsigned ? Unsigned : Signed, NlocalP, NlocalT, N,
                                                 rsT, ciT, rsN, ciN);
+  // the parts' gather+scatter row orders, side by side
+  bool ordered = false;
+  for (const ogsBase_t *p : parts) ordered = ordered || p->gatherLocal->gsRowOrder.length() > 0;
+  if (ordered) {
+    memory<dlong> order(NlocalT);
+    dlong off = 0;
+    for (const ogsBase_t *p : parts) {
+      const ogsOperator_t &op = *p->gatherLocal;
+      const bool has = op.gsRowOrder.length() > 0;
+#pragma omp parallel for schedule(static)
+      for (dlong r = 0; r < op.NrowsT; ++r) order[off + r] = off + (has ? op.gsRowOrder[r] : r);
+      off += op.NrowsT;
+    }
+    gatherLocal->SetGsRowOrder(order);
+  }
   setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

@@ -262,6 +262,17 @@ class Ogs:
         checkh(libspxhost().spxh_ogs_set_gs_format(self.h, 1 if sorted_runs else 0, xcd_remap),
                "spxh_ogs_set_gs_format")
 
+    def set_gs_row_keys(self, keys=None):
+        """Rows of the sorted-run gather+scatter tiles sorted by keys (one int64 per T-CSR row, ties
+        in row order); None restores the CSR's order.  Results do not change."""
+        if keys is None:
+            checkh(libspxhost().spxh_ogs_set_gs_row_keys(self.h, None), "spxh_ogs_set_gs_row_keys")
+            return
+        k = np.ascontiguousarray(keys, dtype=np.int64)
+        if k.size != self.NlocalT:
+            raise ValueError(f"need {self.NlocalT} keys, got {k.size}")
+        checkh(libspxhost().spxh_ogs_set_gs_row_keys(self.h, k.ctypes.data), "spxh_ogs_set_gs_row_keys")
+
     def set_gather_format(self, gather_sorted=None, scatter_sorted=None):
         """T-CSR gather (Trans) and scatter (NoTrans) on the sorted-run blocks (True) or the T-CSR row
         blocks (False); None keeps the current choice (both sorted by default)."""

@@ -384,6 +384,44 @@ def test_signed_ids_non_sym_through_ogs(dev, oracle, dtype, trans, unique):
             g.set_gather_format(True, True)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.int32])
+@pytest.mark.parametrize("source", ["hex", "signed"])
+def test_gs_row_order_does_not_change_results(dev, oracle, dtype, source):
+    """The gather+scatter tiles over rows in any order (random keys, the hex box's 3-D blocks, the CSR's
+    own order) give the oracle's bits for every op and transpose; the gather keeps the CSR's order."""
+    say("gs row order", source, np.dtype(dtype).name)
+    if source == "hex":
+        g = ogs.Ogs.hex(5, 4, 3, 3)                       # 3-D blocks of grid points by default
+        ids = g.ids
+        o = oracle.Ogs(ogs.hex_ids(5, 4, 3, 3))
+        unique = True
+    else:
+        ids = signed_ids(20000, 5, oracle, True)
+        g, o = ogs.Ogs(ids, kind=spx.SIGNED, unique=False), oracle.Ogs(ids, kind=spx.SIGNED, unique=False)
+        unique = False
+    keys = oracle.rand_int(g.NlocalT, 77) % 1000
+    for order in ("default", "random", "csr"):
+        if order == "random":
+            g.set_gs_row_keys(keys)
+        elif order == "csr":
+            g.set_gs_row_keys(None)
+        for op in OPS:
+            for k in (1, 3):
+                q = op_values(g.N, dtype, 40 + k, k, op)
+                for trans in TRANS:
+                    dq = DeviceArray.from_numpy(q)
+                    g.gather_scatter(dq, k, op, trans)
+                    device.check_kernel_errors()
+                    assert same(dq.to_numpy(), o.gather_scatter(q, k, op, trans)), (order, op, k, trans)
+                if unique:
+                    dg = DeviceArray(g.NlocalT * k, np.dtype(dtype))
+                    g.gather(dg, DeviceArray.from_numpy(q), k, op, spx.TRANS)
+                    device.check_kernel_errors()
+                    assert same(dg.to_numpy(), o.gather(q, k, op, spx.TRANS)), (order, op, k)
+    with pytest.raises(ValueError):
+        g.set_gs_row_keys(keys[:-1])
+
+
 def test_batch_non_sym_acts_part_by_part(dev, oracle):
     """A batch whose first part has groups without positive members: the union keeps each part's
     N-CSR rows at its T-CSR rows (empty N rows fill the gap), so the NoTrans / Trans forms act part by
