@@ -4,6 +4,10 @@
 // most `blockNnz` column entries, the LDS tile of the gfx950 kernels.
 #pragma once
 
+#include <omp.h>
+
+#include <vector>
+
 #include "ogs.hpp"
 
 namespace libp {
@@ -165,8 +169,9 @@ class ogsOperator_t {
   void SetBlockNnz(int nnz);
   void SetGsBlockNnz(int nnz);
   // Rows of the gather+scatter tiles in this order (a permutation of the
-  // T-CSR rows; empty: the CSR's); results do not change.
-  void SetGsRowOrder(const memory<dlong> &order);
+  // T-CSR rows; empty: the CSR's); results do not change.  validate: check
+  // that it is a permutation (callers that built one may skip it).
+  void SetGsRowOrder(const memory<dlong> &order, bool validate = true);
   // gsSrb from rowSrb and gsRowOrder, with these launch hints
   void BuildGsSrb(int xcdRemap, int tileLanes);
 
@@ -214,6 +219,38 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
 // Block shape measured best on MI355X for a hex box of degree p (64^3 p=7 and
 // ~1e8-node boxes at p = 1..15).
 void HexGsBlockShape(const int p, int &bx, int &by, int &bz);
+
+// Parallel exclusive prefix sum of v[0..n) in place; returns the total.
+template <typename V>
+inline V ExclusiveScan(V *v, dlong n) {
+  if (n <= 0) return 0;
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  std::vector<V> part(nt + 1, 0);
+#pragma omp parallel num_threads(nt)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    const dlong lo = (dlong)((int64_t)n * t / nt), hi = (dlong)((int64_t)n * (t + 1) / nt);
+    V sum = 0;
+    for (dlong i = lo; i < hi; ++i) sum += v[i];
+    part[t + 1] = sum;
+#pragma omp barrier
+#pragma omp single
+    for (int k = 0; k < nt; ++k) part[k + 1] += part[k];
+    V run = part[t];
+    for (dlong i = lo; i < hi; ++i) {
+      const V x = v[i];
+      v[i] = run;
+      run += x;
+    }
+  }
+  return part[nt];
+}
 
 // Greedy row blocking: consecutive rows, at most `cap` entries per block.
 // Aborts when one row alone exceeds cap (ogsOperator.cpp:535-536).

@@ -98,11 +98,12 @@ void ogsOperator_t::BuildGsSrb(int xcd, int lanes) {
   } else {
     // the T-CSR with its rows in gsRowOrder, cut into greedy blocks
     memory<dlong> rs(NrowsT + 1), ci(nnzT > 0 ? nnzT : 1);
-    rs[0] = 0;
+#pragma omp parallel for schedule(static)
     for (dlong i = 0; i < NrowsT; ++i) {
       const dlong r = gsRowOrder[i];
-      rs[i + 1] = rs[i] + (rowStartsT[r + 1] - rowStartsT[r]);
+      rs[i] = rowStartsT[r + 1] - rowStartsT[r];
     }
+    rs[NrowsT] = ExclusiveScan(rs.ptr(), NrowsT);
 #pragma omp parallel for schedule(static)
     for (dlong i = 0; i < NrowsT; ++i) {
       const dlong r = gsRowOrder[i];
@@ -118,8 +119,8 @@ void ogsOperator_t::BuildGsSrb(int xcd, int lanes) {
   gsSrb.tileLanes = lanes;
 }
 
-void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order) {
-  if (order.length()) {
+void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order, const bool validate) {
+  if (order.length() && validate) {
     LIBP_ABORT("gather+scatter row order must list the " << NrowsT << " rows", (dlong)order.length() != NrowsT);
     std::vector<char> seen(NrowsT, 0);
     for (dlong i = 0; i < NrowsT; ++i) {
@@ -176,15 +177,20 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
 }
 
 void HexGsBlockShape(const int p, int &bx, int &by, int &bz) {
-  // 64^3 p=7 and ~1e8-node boxes (us, default order -> blocks): p=1 640 -> 436
-  // with 8x4x4, p=2 601 -> 447 with 8x8x4, p=3 387 -> 361 and p=7 470 -> 461,
-  // p=15 357 -> 312 with 16x8x2
+  // Measured on MI355X, us per call (CSR order -> blocks): ~1e8 nodes, FP64:
+  // p=1 640 -> 436 (8x4x4), p=2 601 -> 447 (8x8x4), p=3 387 -> 361 and
+  // p=4 415 -> 410 (16x8x2), p=11 383 -> 342 and p=15 357 -> 303 (64x4x1);
+  // 64^3 p=7, FP64 / FP32 / BF16 storage (medians of 3 interleaved rounds):
+  // 469 / 347 / 370 -> 453 / 341 / 368 (64x8x1), where 64x4x1 gives
+  // 460 / 335 / 357 and 16x8x2 460 / 366 / 393.
   if (p <= 1) {
     bx = 8; by = 4; bz = 4;
   } else if (p == 2) {
     bx = 8; by = 8; bz = 4;
-  } else {
+  } else if (p <= 5) {
     bx = 16; by = 8; bz = 2;
+  } else {
+    bx = 64; by = 8; bz = 1;
   }
 }
 

@@ -49,37 +49,6 @@ void lcgInts(int64_t n, uint64_t seed, std::vector<int32_t> &out) {
 
 }  // namespace
 
-// Parallel exclusive prefix sum of v[0..n) in place; returns the total.
-template <typename V>
-static V ExclusiveScan(V *v, dlong n) {
-  if (n <= 0) return 0;
-  int nt = 1;
-#ifdef _OPENMP
-  nt = omp_get_max_threads();
-#endif
-  std::vector<V> part(nt + 1, 0);
-#pragma omp parallel num_threads(nt)
-  {
-    int t = 0;
-#ifdef _OPENMP
-    t = omp_get_thread_num();
-#endif
-    const dlong lo = (dlong)((int64_t)n * t / nt), hi = (dlong)((int64_t)n * (t + 1) / nt);
-    V sum = 0;
-    for (dlong i = lo; i < hi; ++i) sum += v[i];
-    part[t + 1] = sum;
-#pragma omp barrier
-#pragma omp single
-    for (int k = 0; k < nt; ++k) part[k + 1] += part[k];
-    V run = part[t];
-    for (dlong i = lo; i < hi; ++i) {
-      const V x = v[i];
-      v[i] = run;
-      run += x;
-    }
-  }
-  return part[nt];
-}
 
 LocalCSR BuildLocalCSR(const dlong N, memory<hlong> ids, const Kind kind, const bool unique, const int rank) {
   LIBP_ABORT("Invalid ogs setup requested", (kind == Unsigned && unique) || (kind == Halo && unique));
@@ -320,7 +289,7 @@ void ogsBase_t::SetupHexBox(const dlong nx, const dlong ny, const dlong nz, cons
     }
     int bx, by, bz;
     HexGsBlockShape(p, bx, by, bz);
-    gatherLocal->SetGsRowOrder(LatticeBlockOrder(op.NrowsT, grid, Mx, My, Mz, bx, by, bz));
+    gatherLocal->SetGsRowOrder(LatticeBlockOrder(op.NrowsT, grid, Mx, My, Mz, bx, by, bz), false);
   }
   setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -412,7 +381,7 @@ void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform
       for (dlong r = 0; r < op.NrowsT; ++r) order[off + r] = off + (has ? op.gsRowOrder[r] : r);
       off += op.NrowsT;
     }
-    gatherLocal->SetGsRowOrder(order);
+    gatherLocal->SetGsRowOrder(order, false);
   }
   setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
