@@ -35,6 +35,7 @@ typedef struct {
   int32_t kind, unique;
   int64_t NgatherGlobal;
   double setup_seconds;
+  int32_t scatter_by_rows; /* scatter path: 1 row blocks, 0 local order (spxh_ogs_set_scatter_path) */
 } spxh_ogs_info_t;
 
 const char *spxh_last_error(void);
@@ -68,8 +69,10 @@ int spxh_ogs_copy_csr(spxh_ogs_t ogs, int which, int32_t *rowStarts, int32_t *co
    blocks, and gather+scatter blocks */
 int spxh_ogs_set_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
 int spxh_ogs_set_gs_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
-/* scatter path: 1 = over the CSR row blocks (ogsKernels.okl scatter; the
-   default), 0 = through the local->row map in local order */
+/* scatter path: 1 = over the row blocks (ogsKernels.okl scatter), 0 =
+   through the local->row map in local order; results are identical.  The
+   setup picks local order when the operator has fewer than 0.63 rows per
+   local entry (short rows: there its coalesced stores win), else rows. */
 int spxh_ogs_set_scatter_path(spxh_ogs_t ogs, int by_rows);
 /* layout of the Sym gather+scatter: 1 = sorted-run row blocks (spx_srb_t, the
    default), 0 = the T-CSR row blocks; results are identical.  xcd_remap is the

@@ -54,7 +54,8 @@ class OgsInfo(C.Structure):
     _fields_ = [("N", i32), ("Ngather", i32), ("NlocalT", i32), ("NlocalP", i32), ("nnzT", i32), ("nnzN", i32),
                 ("NblocksT", i32), ("NblocksN", i32), ("block_nnz", i32), ("gs_block_nnz", i32),
                 ("Ngs_blocks", i32), ("gather_defined", i32),
-                ("kind", i32), ("unique", i32), ("NgatherGlobal", i64), ("setup_seconds", dbl)]
+                ("kind", i32), ("unique", i32), ("NgatherGlobal", i64), ("setup_seconds", dbl),
+                ("scatter_by_rows", i32)]
 
 
 class GsTuning(C.Structure):

@@ -300,7 +300,7 @@ int spx_ogs_scatter_inverse(int type, int K, int32_t n, const int32_t *g2l, cons
   int rc = by_type(type, [&](auto *tp) {
     using T = std::remove_pointer_t<decltype(tp)>;
     spx_tuning_t t = tuning_for("ogs_scatter_inverse", spx_tuning_t{});
-    return launch_scatter_inv(current_stream(), (int64_t)n, K, g2l, (const T *)gv, (T *)v, t.unroll > 0 ? t.unroll : 8);
+    return launch_scatter_inv(current_stream(), (int64_t)n, K, g2l, (const T *)gv, (T *)v, t.unroll > 0 ? t.unroll : 4);
   });
   if (rc) return rc;
   SPX_LAUNCHED("spx_ogs_scatter_inverse");

@@ -143,9 +143,13 @@ class ogsOperator_t {
   memory<uint8_t> memberFlagN;  // per local node: N-CSR member (signedSrb only)
 
   // Local -> gathered row maps of the T- and N-CSR (-1: in no row), for the
-  // scatter in local order (spx_ogs_scatter_inverse).  Measured on MI355X at
-  // 64^3, p=7 the CSR-order scatter is faster, so it is the default.
+  // scatter in local order (spx_ogs_scatter_inverse), whose stores are
+  // coalesced whatever the rows.  It is the default when rows are short: hex
+  // boxes at ~1e8 nodes, us per call (row blocks / local order): p=1 553 /
+  // 236, p=3 325 / 291, p=5 328 / 288, but p=7 394 / 405 and p=15 326 / 347,
+  // with rows / local nodes 0.13, 0.43, 0.58, 0.68 and 0.83.
   deviceMemory<dlong> o_localRowT, o_localRowN;
+  static constexpr double kScatterByRowsMinRatio = 0.63;
   bool scatterByRows = true;  // false: scatter through the local -> row maps
 
   // The non-symmetric gather+scatter stages two value tiles, so its row

@@ -138,6 +138,7 @@ int spxh_ogs_info(spxh_ogs_t h, spxh_ogs_info_t *info) {
     info->unique = o.unique ? 1 : 0;
     info->NgatherGlobal = o.NgatherGlobal;
     info->setup_seconds = o.setupSeconds();
+    info->scatter_by_rows = op->scatterByRows ? 1 : 0;
   });
 }
 

@@ -74,6 +74,7 @@ ogsOperator_t::ogsOperator_t(platform_t &platform_, Kind kind_, const dlong Nrow
   };
   o_localRowT = localRows(NrowsT, rowStartsT, colIdsT);
   o_localRowN = (kind == Signed) ? localRows(NrowsN, rowStartsN, colIdsN) : o_localRowT;
+  scatterByRows = NrowsT >= kScatterByRowsMinRatio * Ncols;
 }
 
 void ogsOperator_t::SetGsBlockNnz(int nnz) {

@@ -551,6 +551,7 @@ def test_scatter_paths_agree(setups, dtype, trans, k):
     gsrc = values(srows, dtype, 8, k)
     base = values(g.N, dtype, 9, k)
     want = o.scatter(gsrc, base, k, trans)
+    default = g.info["scatter_by_rows"]
     try:
         for by_rows in (False, True):
             g.set_scatter_path(by_rows)
@@ -559,7 +560,7 @@ def test_scatter_paths_agree(setups, dtype, trans, k):
             device.check_kernel_errors()
             assert same(dv.to_numpy(), want), by_rows
     finally:
-        g.set_scatter_path(True)
+        g.set_scatter_path(bool(default))
 
 
 def test_unflagged_scatter_leaves_zero_ids_alone(dev, oracle):
@@ -576,7 +577,15 @@ def test_unflagged_scatter_leaves_zero_ids_alone(dev, oracle):
         g.scatter(dv, dg, 1, spx.NOTRANS)
         assert same(dv.to_numpy(), want)
         assert same(dv.to_numpy()[::7], base[::7])
-    g.set_scatter_path(True)
+
+
+@pytest.mark.parametrize("p,by_rows", [(1, False), (2, False), (3, False), (7, True), (11, True)])
+def test_scatter_path_default_follows_row_length(dev, p, by_rows):
+    """Short rows (fewer than 0.63 rows per local node) scatter in local order, long rows by rows."""
+    g = ogs.Ogs.hex(3, 3, 3, p, keep_ids=False)
+    ratio = g.NlocalT / g.N
+    say("scatter default", p, round(ratio, 3))
+    assert bool(g.info["scatter_by_rows"]) == by_rows == (ratio >= 0.63)
 
 
 @pytest.mark.parametrize("mesh", [(2, 2, 1, 2), (3, 2, 2, 3)])
