@@ -140,8 +140,8 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
   const hlong nbx = (Mx + bx - 1) / bx, nby = (My + by - 1) / by, nbz = (Mz + bz - 1) / bz;
   const hlong nkeys = nbx * nby * nbz;
   LIBP_ABORT("too many lattice blocks (" << nkeys << ")", nkeys > (hlong)INT32_MAX);
-  memory<dlong> order(Nrows > 0 ? Nrows : 1);
-  if (Nrows == 0) return order;
+  if (Nrows <= 0) return memory<dlong>();  // no rows: the CSR's (empty) order
+  memory<dlong> order(Nrows);
   std::vector<int32_t> key(Nrows);
 #pragma omp parallel for schedule(static)
   for (dlong r = 0; r < Nrows; ++r) {
