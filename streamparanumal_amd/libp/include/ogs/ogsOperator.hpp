@@ -192,8 +192,13 @@ class ogsOperator_t {
   void GatherScatterMixedRaw(int storage, int k, void *v);
   // Time every candidate launch shape of the Sym gather+scatter for values of
   // `type` (storage < 0) or of mixed storage `storage`, k fields; keep and
-  // return the fastest.
+  // return the fastest if it beats the current shape by more than
+  // 1 - kAutotuneMargin, else keep (and return) the current one.
   GsTuning Autotune(int type, int storage, int k, int reps, std::vector<GsTuning> *log = nullptr);
+  // the candidates are timed on a scratch vector, and placement alone moved a
+  // shape's time by up to 4 % between it and the caller's vector (Tet p=3:
+  // 639 us on scratch, 640 on the data, against 625 for the current shape)
+  static constexpr double kAutotuneMargin = 0.95;
 };
 
 // The local operator's CSR pair as built by the setup (host arrays).
