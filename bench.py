@@ -248,8 +248,9 @@ def precision_errors(alpha=0.31, seed=3):
 
 def element_types(K, W, seed=2):
     """FP64 gather (Trans), scatter (NoTrans) and gather+scatter (Sym) on a box of each element type
-    at p=3 and p=7, sized to about 1e8 local nodes, set up through the general ogs setup from the
-    box ids (the hex box through its closed form)."""
+    at p=3 and p=7, sized to about 1e8 local nodes, set up as the mesh does (Ogs.box: the general
+    ogs setup from the box ids, the hex box in closed form, and the mesh's gather+scatter tile
+    order)."""
     import streamparanumal_amd as spx
     from streamparanumal_amd import device, ogs
     from streamparanumal_amd.device import DeviceArray
@@ -263,10 +264,7 @@ def element_types(K, W, seed=2):
             n = int(round((1e8 / per_cell) ** (1.0 / dim)))
             nz = n if dim == 3 else 1
             t0 = time.perf_counter()
-            if el == "Hex":
-                g = ogs.Ogs.hex(n, n, n, p, keep_ids=False)
-            else:
-                g = ogs.Ogs(ogs.box_ids(el, n, n, nz, p))
+            g = ogs.Ogs.box(el, n, n, nz, p, keep_ids=False)
             setup = time.perf_counter() - t0
             Nl, Ng = g.N, g.NgatherGlobal
             q = DeviceArray(Nl)

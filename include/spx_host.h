@@ -57,6 +57,15 @@ int spxh_ogs_info(spxh_ogs_t ogs, spxh_ogs_info_t *info);
    would leave them. */
 int spxh_ogs_setup_hex(int32_t nx, int32_t ny, int32_t nz, int p, int kind, int unique, int64_t *ids,
                        spxh_ogs_t *out);
+/* mesh_t's ogs_t of an element-type box (SPXH_TRIANGLES ... SPXH_HEXAHEDRA, 2-D types
+   ignore nz; libs/mesh/meshGatherScatterSetup.cpp:31-44 over the ids of
+   spxh_mesh_box_ids): the same operator as spxh_ogs_setup over those ids,
+   with the gather+scatter tile order the mesh uses (lattice blocks for hex
+   boxes and for Tri / Quad boxes up to degree 3).  ids, when not NULL, has
+   room for the box's elements x nodes per element (spxh_mesh_box_shape) and
+   receives the sign-flagged ids. */
+int spxh_ogs_setup_box(int type, int32_t nx, int32_t ny, int32_t nz, int p, int kind, int unique, int64_t *ids,
+                       spxh_ogs_t *out);
 /* The block-diagonal union of nparts set-up handles as one new handle (no
    reference counterpart): part i's local nodes follow parts 0..i-1's, so a
    batch of independent meshes is gathered in one launch.  The parts stay

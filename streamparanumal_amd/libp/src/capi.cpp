@@ -95,6 +95,28 @@ int spxh_ogs_setup_hex(int32_t nx, int32_t ny, int32_t nz, int p, int kind, int 
   });
 }
 
+int spxh_ogs_setup_box(int type, int32_t nx, int32_t ny, int32_t nz, int p, int kind, int unique, int64_t *ids,
+                       spxh_ogs_t *out) {
+  if (!out) return err(SPX_ERR_ARG, "spxh_ogs_setup_box: null out");
+  if (!state().open) return err(SPX_ERR_NODEVICE, "spxh_ogs_setup_box: call spxh_platform_open first");
+  if (kind < 0 || kind > 2) return err(SPX_ERR_ARG, "spxh_ogs_setup_box: kind must be 0, 1 or 2");
+  *out = nullptr;
+  return guarded([&] {
+    const int Np = mesh_t::NodesPerElement(type, p);
+    LIBP_ABORT("spxh_ogs_setup_box: unknown element type or degree", Np <= 0 || p < 1);
+    LIBP_ABORT("spxh_ogs_setup_box: box sizes must be positive", nx < 1 || ny < 1 || nz < 1);
+    const long long n = mesh_t::BoxElements(type, nx, ny, nz) * Np;
+    LIBP_ABORT("spxh_ogs_setup_box: " << n << " local nodes exceed the int32 range", n >= (long long)INT32_MAX);
+    memory<hlong> lids((size_t)n);
+    if (type != mesh_t::HEXAHEDRA) mesh_t::BoxGlobalIds(type, nx, ny, nz, p, lids.ptr());
+    auto h = std::make_unique<spxh_ogs_s>();
+    mesh_t::BoxGatherScatterSetup(h->ogs, type, nx, ny, nz, p, lids, comm_t::world(), state().platform,
+                                  (ogs::Kind)kind, unique != 0);
+    if (ids) std::copy(lids.ptr(), lids.ptr() + n, ids);
+    *out = h.release();
+  });
+}
+
 int spxh_ogs_setup_batch(const spxh_ogs_t *parts, int32_t nparts, spxh_ogs_t *out) {
   if (!out || !parts || nparts < 1) return err(SPX_ERR_ARG, "spxh_ogs_setup_batch: bad arguments");
   if (!state().open) return err(SPX_ERR_NODEVICE, "spxh_ogs_setup_batch: call spxh_platform_open first");

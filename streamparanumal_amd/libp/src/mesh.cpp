@@ -219,15 +219,37 @@ void mesh_t::SetupBox() {
 }
 
 // A hex box's operator is built in closed form (ogs_t::SetupHexBox): the same
-// operator, and the same sign-flagged ids, as ogs.Setup over globalIds.  The
+// operator, and the same sign-flagged ids, as ogs.Setup over its ids.  The
 // other element types go through ogs.Setup.
+void mesh_t::BoxGatherScatterSetup(ogs::ogs_t &ogs, int type, dlong nx, dlong ny, dlong nz, int N,
+                                   memory<hlong> ids, comm_t comm, platform_t &platform, ogs::Kind kind,
+                                   bool unique) {
+  if (type == HEXAHEDRA) {
+    ogs.SetupHexBox(nx, ny, nz, N, comm, kind, unique, platform, ids.ptr());
+    return;
+  }
+  ogs.Setup((dlong)ids.length(), ids, comm, kind, ogs::Auto, unique, false, platform);
+  if ((type == TRIANGLES || type == QUADRILATERALS) && N <= 3) {
+    // gather+scatter tiles over 12 x 12 blocks of the node lattice (the ids
+    // are lattice ids, 1 + gx + gy*Gx): measured on MI355X at ~1e8 nodes,
+    // p=3, us per call, Quad 379 -> 347, Tri 394 -> 368; at p=7 and for
+    // tetrahedra the CSR's order was as fast or faster
+    auto op = ogs.localOperator();
+    const hlong Gx = (hlong)nx * N + 1, Gy = (hlong)ny * N + 1;
+    memory<hlong> grid(op->NrowsT > 0 ? op->NrowsT : 1);
+#pragma omp parallel for schedule(static)
+    for (dlong r = 0; r < op->NrowsT; ++r) {
+      const hlong id = ids[op->colIdsT[op->rowStartsT[r]]];
+      grid[r] = (id < 0 ? -id : id) - 1;
+    }
+    op->SetGsRowOrder(ogs::LatticeBlockOrder(op->NrowsT, grid, Gx, Gy, 1, 12, 12, 1), false);
+  }
+}
+
 void mesh_t::GatherScatterSetup() {
   const bool verbose = settings.compareSetting("VERBOSE", "TRUE");
   const bool unique = true;  // flag a unique node in every gather node
-  if (elementType == HEXAHEDRA)
-    ogs.SetupHexBox(nx, ny, nz, N, comm, ogs::Signed, unique, platform, globalIds.ptr());
-  else
-    ogs.Setup((dlong)Nelements * Np, globalIds, comm, ogs::Signed, ogs::Auto, unique, false, platform);
+  BoxGatherScatterSetup(ogs, elementType, nx, ny, nz, N, globalIds, comm, platform, ogs::Signed, unique);
   if (verbose && comm.rank() == 0)
     std::cout << "ogs Setup: " << ogs.NlocalT << " gather nodes; setup " << ogs.setupSeconds() << " seconds."
               << std::endl;

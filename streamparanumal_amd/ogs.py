@@ -220,6 +220,22 @@ class Ogs:
         return self
 
     @classmethod
+    def box(cls, element, nx, ny, nz, p, kind=_lib.SIGNED, unique=True, keep_ids=True, device_id=0):
+        """mesh_t's ogs_t of a box of `element` ("Tri", "Quad", "Tet", "Hex"): the operator of
+        Ogs(box_ids(...)) with the gather+scatter tile order the mesh uses."""
+        platform_open(device_id)
+        self = cls.__new__(cls)
+        ne, npe = box_shape(element, nx, ny, nz, p)
+        ids = np.empty(ne * npe, dtype=np.int64) if keep_ids else None
+        self.h = C.c_void_p()
+        checkh(libspxhost().spxh_ogs_setup_box(ELEMENT_TYPES[element], nx, ny, nz, p, kind, 1 if unique else 0,
+                                               ids.ctypes.data if keep_ids else None, C.byref(self.h)),
+               "spxh_ogs_setup_box")
+        self.ids = ids
+        self.refresh()
+        return self
+
+    @classmethod
     def batch(cls, parts):
         """The block-diagonal union of set-up handles: part i's local nodes follow parts 0..i-1's."""
         self = cls.__new__(cls)

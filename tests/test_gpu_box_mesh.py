@@ -80,3 +80,35 @@ def test_drivers_run_every_element_type(bs, el, dims):
     nz = int(dims[5]) if el == "Tet" else 0
     want = (nx * 4 + 1) * (ny * 4 + 1) * ((nz * 4 + 1) if nz else 1)
     assert int(dofs) == want
+
+
+@pytest.mark.parametrize("element,nx,ny,nz,p", [("Tri", 30, 21, 1, 2), ("Tri", 17, 13, 1, 3), ("Quad", 40, 25, 1, 1),
+                                                ("Quad", 26, 19, 1, 3), ("Quad", 11, 9, 1, 6), ("Tet", 5, 4, 3, 3),
+                                                ("Hex", 6, 5, 4, 2)])
+def test_mesh_box_setup(dev, oracle, element, nx, ny, nz, p):
+    """mesh_t's setup (lattice-block gather+scatter tiles for hex boxes and Tri / Quad up to p = 3):
+    the operator and flags of the general setup over the box ids, and the oracle's bits for every
+    transpose of the gather+scatter, the gather and the scatter."""
+    say("mesh box", element, nx, ny, nz, p)
+    ids = ogs.box_ids(element, nx, ny, nz, p)
+    g, ref, o = ogs.Ogs.box(element, nx, ny, nz, p), ogs.Ogs(ids), oracle.Ogs(ids)
+    assert np.array_equal(g.ids, ref.ids)
+    for which in ("T", "N"):
+        for a, b in zip(g.csr(which), ref.csr(which)):
+            assert np.array_equal(a, b)
+    for k in (1, 2):
+        q = oracle.uniform(g.N * k, 4)
+        for trans in (spx.SYM, spx.NOTRANS, spx.TRANS):
+            dq = DeviceArray.from_numpy(q)
+            g.gather_scatter(dq, k, spx.ADD, trans)
+            device.check_kernel_errors()
+            assert same(dq.to_numpy(), o.gather_scatter(q, k, spx.ADD, trans)), (k, trans)
+        dg = DeviceArray(g.NlocalT * k)
+        g.gather(dg, DeviceArray.from_numpy(q), k, spx.ADD, spx.TRANS)
+        gq = dg.to_numpy()
+        assert same(gq, o.gather(q, k, spx.ADD, spx.TRANS))
+        dv = DeviceArray(g.N * k)
+        dv.fill_zero()
+        g.scatter(dv, dg, k, spx.NOTRANS)
+        device.check_kernel_errors()
+        assert same(dv.to_numpy(), o.scatter(gq, np.zeros(g.N * k), k, spx.NOTRANS))
