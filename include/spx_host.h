@@ -61,7 +61,8 @@ int spxh_ogs_setup_hex(int32_t nx, int32_t ny, int32_t nz, int p, int kind, int 
    ignore nz; libs/mesh/meshGatherScatterSetup.cpp:31-44 over the ids of
    spxh_mesh_box_ids): the same operator as spxh_ogs_setup over those ids,
    with the gather+scatter tile order the mesh uses (lattice blocks for hex
-   boxes and for Tri / Quad boxes up to degree 3).  ids, when not NULL, has
+   boxes, Tri / Quad boxes up to degree 3 and Tet boxes of degree 1 and 3).
+   ids, when not NULL, has
    room for the box's elements x nodes per element (spxh_mesh_box_shape) and
    receives the sign-flagged ids. */
 int spxh_ogs_setup_box(int type, int32_t nx, int32_t ny, int32_t nz, int p, int kind, int unique, int64_t *ids,

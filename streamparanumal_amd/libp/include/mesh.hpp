@@ -61,9 +61,9 @@ class mesh_t {
   // degree-N lattice (nx*N+1) x (ny*N+1) [x (nz*N+1)], for any element type.
   static void BoxGlobalIds(int type, dlong nx, dlong ny, dlong nz, int N, hlong *ids);
   // ogs_t of a box (meshGatherScatterSetup.cpp:31-44): ids hold BoxGlobalIds
-  // (filled here for hexahedra) and receive the sign flags.  Hex boxes, and
-  // Tri / Quad boxes up to degree 3, order their gather+scatter tiles by
-  // blocks of lattice points.
+  // (filled here for hexahedra) and receive the sign flags.  Hex boxes, Tri /
+  // Quad boxes up to degree 3 and Tet boxes of degree 1 and 3 order their
+  // gather+scatter tiles by blocks of lattice points.
   static void BoxGatherScatterSetup(ogs::ogs_t &ogs, int type, dlong nx, dlong ny, dlong nz, int N,
                                     memory<hlong> ids, comm_t comm, platform_t &platform, ogs::Kind kind,
                                     bool unique);

@@ -233,20 +233,31 @@ void mesh_t::BoxGatherScatterSetup(ogs::ogs_t &ogs, int type, dlong nx, dlong ny
   // scatter runs in local order (p=7, ~1e8 nodes: 323 -> 304 us), which the
   // row ratio alone does not pick at p=7 (0.68 rows per node)
   if (type == TRIANGLES) ogs.localOperator()->scatterByRows = false;
+  // gather+scatter tiles over blocks of the node lattice (the ids are lattice
+  // ids, 1 + gx + gy*Gx [+ gz*Gx*Gy]) where that measured faster on MI355X at
+  // ~1e8 nodes, us per call: Quad p=3 379 -> 347 and Tri p=3 394 -> 368 with
+  // 12 x 12 (p=1, 2 likewise short-run); Tet p=1 513 -> 414 with 4 x 2 x 2,
+  // Tet p=3 626 -> 576 with 6 x 6 x 6.  At 2-D p=7 and Tet p=2 or 7 the CSR's
+  // order was as fast or faster.
+  int bx = 0, by = 0, bz = 0;
   if ((type == TRIANGLES || type == QUADRILATERALS) && N <= 3) {
-    // gather+scatter tiles over 12 x 12 blocks of the node lattice (the ids
-    // are lattice ids, 1 + gx + gy*Gx): measured on MI355X at ~1e8 nodes,
-    // p=3, us per call, Quad 379 -> 347, Tri 394 -> 368; at p=7 and for
-    // tetrahedra the CSR's order was as fast or faster
+    bx = 12; by = 12; bz = 1;
+  } else if (type == TETRAHEDRA && N == 1) {
+    bx = 4; by = 2; bz = 2;
+  } else if (type == TETRAHEDRA && N == 3) {
+    bx = 6; by = 6; bz = 6;
+  }
+  if (bx) {
     auto op = ogs.localOperator();
     const hlong Gx = (hlong)nx * N + 1, Gy = (hlong)ny * N + 1;
+    const hlong Gz = type == TETRAHEDRA ? (hlong)nz * N + 1 : 1;
     memory<hlong> grid(op->NrowsT > 0 ? op->NrowsT : 1);
 #pragma omp parallel for schedule(static)
     for (dlong r = 0; r < op->NrowsT; ++r) {
       const hlong id = ids[op->colIdsT[op->rowStartsT[r]]];
       grid[r] = (id < 0 ? -id : id) - 1;
     }
-    op->SetGsRowOrder(ogs::LatticeBlockOrder(op->NrowsT, grid, Gx, Gy, 1, 12, 12, 1), false);
+    op->SetGsRowOrder(ogs::LatticeBlockOrder(op->NrowsT, grid, Gx, Gy, Gz, bx, by, bz), false);
   }
 }
 

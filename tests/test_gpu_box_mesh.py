@@ -84,9 +84,9 @@ def test_drivers_run_every_element_type(bs, el, dims):
 
 @pytest.mark.parametrize("element,nx,ny,nz,p", [("Tri", 30, 21, 1, 2), ("Tri", 17, 13, 1, 3), ("Quad", 40, 25, 1, 1),
                                                 ("Quad", 26, 19, 1, 3), ("Quad", 11, 9, 1, 6), ("Tet", 5, 4, 3, 3),
-                                                ("Hex", 6, 5, 4, 2)])
+                                                ("Tet", 7, 6, 5, 1), ("Tet", 4, 3, 3, 2), ("Hex", 6, 5, 4, 2)])
 def test_mesh_box_setup(dev, oracle, element, nx, ny, nz, p):
-    """mesh_t's setup (lattice-block gather+scatter tiles for hex boxes and Tri / Quad up to p = 3):
+    """mesh_t's setup (lattice-block gather+scatter tiles for hex boxes, Tri / Quad up to p = 3, Tet p = 1, 3):
     the operator and flags of the general setup over the box ids, and the oracle's bits for every
     transpose of the gather+scatter, the gather and the scatter."""
     say("mesh box", element, nx, ny, nz, p)
