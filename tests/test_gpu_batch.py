@@ -46,6 +46,32 @@ def test_union_gather_scatter_per_instance(parts, oracle, st):
         assert np.array_equal(got[a:e].view(np.uint8), want.view(np.uint8))
 
 
+@pytest.mark.parametrize("st", [STORE_F32, STORE_F64])
+def test_union_of_ordered_parts(dev, oracle, st):
+    """Parts from the closed-form hex setup carry their lattice-block tile orders; the union takes
+    them side by side, and its gather+scatter still gives each instance's oracle bits."""
+    say("union of ordered parts", st)
+    b = Batch(MESHES, K=3, dtype=NP[st])                 # parts from Ogs.hex
+    q64 = oracle.uniform(3 * b.N, 8)
+    q = oracle.to_storage(q64, st) if st != STORE_F64 else q64
+    dq = DeviceArray.from_numpy(q)
+    b.union.gather_scatter_mixed(dq, 3)
+    device.check_kernel_errors()
+    got = dq.to_numpy()
+    for (a, e), (n, p) in zip(slices(b), MESHES):
+        o = oracle.Ogs(ogs.hex_ids(n, n, n, p))
+        want = oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, q[a:e], 3)
+        assert np.array_equal(got[a:e].view(np.uint8), want.view(np.uint8)), (n, p)
+    if st == STORE_F64:
+        dq = DeviceArray.from_numpy(q64)
+        b.union.gather_scatter(dq, 3)
+        device.check_kernel_errors()
+        got = dq.to_numpy()
+        for (a, e), (n, p) in zip(slices(b), MESHES):
+            o = oracle.Ogs(ogs.hex_ids(n, n, n, p))
+            assert np.array_equal(got[a:e].view(np.uint64), o.gather_scatter(q64[a:e], 3).view(np.uint64))
+
+
 @pytest.mark.parametrize("st", [STORE_F32, STORE_BF16, STORE_F64])
 @pytest.mark.parametrize("blocks", [1, 7, 64, 1024])
 def test_cg_batch_per_segment(dev, oracle, st, blocks):
