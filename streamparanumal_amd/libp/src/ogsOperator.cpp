@@ -183,10 +183,12 @@ void HexGsBlockShape(const int p, int &bx, int &by, int &bz) {
   // p=4 415 -> 410 (16x8x2), p=11 383 -> 342 and p=15 357 -> 303 (64x4x1);
   // 64^3 p=7, FP64 / FP32 / BF16 storage (medians of 3 interleaved rounds):
   // 469 / 347 / 370 -> 453 / 341 / 368 (64x8x1), where 64x4x1 gives
-  // 460 / 335 / 357 and 16x8x2 460 / 366 / 393.
+  // 460 / 335 / 357 and 16x8x2 460 / 366 / 393.  From p=5 up the shapes
+  // tried (8x8x4 to 128x4x1) differ by at most 3 %; p=4 8x8x4 398 against
+  // 410 for 16x8x2.
   if (p <= 1) {
     bx = 8; by = 4; bz = 4;
-  } else if (p == 2) {
+  } else if (p == 2 || p == 4) {
     bx = 8; by = 8; bz = 4;
   } else if (p <= 5) {
     bx = 16; by = 8; bz = 2;
