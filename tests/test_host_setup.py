@@ -210,6 +210,21 @@ def test_srb_signed_rejects_misaligned_n_csr():
             ogs.srb_build(rs, ci, bs, 64, n_csr=args, ncols=4)
 
 
+def test_srb_rows_ascending_flag():
+    """The layout numbers rows by their leading positions, so it reports whether each tile's rows
+    ascend by first member (and have one): the gather and scatter need it, the gather+scatter not."""
+    bs = np.array([0, 2], np.int32)
+    up = ogs.srb_build(np.array([0, 2, 3], np.int32), np.array([0, 2, 1], np.int32), bs, 64)
+    assert up["rows_ascending"]                                   # first members 0 < 1
+    down = ogs.srb_build(np.array([0, 1, 3], np.int32), np.array([2, 0, 1], np.int32), bs, 64)
+    assert not down["rows_ascending"]                             # first members 2 > 0
+    split = ogs.srb_build(np.array([0, 1, 3], np.int32), np.array([2, 0, 1], np.int32),
+                          np.array([0, 1, 2], np.int32), 64)
+    assert split["rows_ascending"]                                # one row per tile: nothing to order
+    empty = ogs.srb_build(np.array([0, 2, 2], np.int32), np.array([0, 1], np.int32), bs, 64)
+    assert not empty["rows_ascending"]                            # an empty row has no leading position
+
+
 def test_srb_rejects_bad_blocks():
     rs = np.array([0, 2, 3], np.int32)
     ci = np.array([0, 1, 2], np.int32)
