@@ -368,6 +368,9 @@ def main():
     ap.add_argument("--only-batched", action="store_true",
                     help="run the 64-mesh batched comparison alone and print its JSON: the command the committed "
                          "rocprofv3 summary of the batched kernels is taken from")
+    ap.add_argument("--only-degree-sweep", action="store_true",
+                    help="run the p=1..15 sweep at ~1e8 nodes alone and print its JSON: the command the committed "
+                         "rocprofv3 summary of the sweep's kernels is taken from")
     ap.add_argument("--only-bs8", action="store_true",
                     help="time the BS8 step alone (no suite, no CPU baseline): the command the committed "
                          "rocprofv3 summaries of the BS8 kernel are taken from")
@@ -389,6 +392,9 @@ def main():
     log("device", props["name"], props["arch"], props["compute_units"], "CUs")
     if args.only_batched:
         print(json.dumps({"batched_64_meshes": batched_suite(K, W)}), flush=True)
+        return
+    if args.only_degree_sweep:
+        print(json.dumps({"degree_sweep_1e8": degree_sweep(K, W)}), flush=True)
         return
 
     # ---------------------------------------------------------------- BS6-BS8
