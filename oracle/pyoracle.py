@@ -39,6 +39,8 @@ _FUNCS = {
     "oracle_ogs_scatter": (None, [C.POINTER(OgsStruct), C.c_int, C.c_int, C.c_int, vp, vp]),
     "oracle_ogs_gather_scatter": (None, [C.POINTER(OgsStruct), C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "oracle_csr_gs": (None, [i32, vp, vp, C.c_int, vp]),
+    "oracle_csr_gather": (None, [i32, vp, vp, C.c_int, vp, vp]),
+    "oracle_csr_scatter": (None, [i32, vp, vp, C.c_int, vp, vp]),
     "oracle_ogs_gs_okl": (None, [i32, C.c_int, vp, vp, vp, vp, vp, vp]),
     "oracle_bf16_from_double": (C.c_uint16, [dbl]),
     "oracle_bf16_from_float": (C.c_uint16, [C.c_float]),
@@ -47,6 +49,8 @@ _FUNCS = {
     "oracle_cg_mixed": (dbl, [C.c_int, i64, dbl, vp, vp, vp, vp]),
     "oracle_time_bs8": (dbl, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(dbl),
                               C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "oracle_time_ogs": (None, [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.POINTER(dbl),
+                               C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "oracle_time_stream": (dbl, [C.c_int, i64, C.c_int, C.c_int]),
     "oracle_max_threads": (C.c_int, []),
 }
@@ -215,6 +219,27 @@ def csr_gs(rowStarts, colIds, q, k=1):
     return q
 
 
+def csr_gather(rowStarts, colIds, q, k=1):
+    """Add gather of q over the given CSR: gv[row] = sum of q over the row's columns in stored order
+    (the host loop of ogsOperator.cpp:61-114)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    rs = np.ascontiguousarray(rowStarts, dtype=np.int32)
+    ci = np.ascontiguousarray(colIds, dtype=np.int32)
+    gv = np.empty((rs.size - 1) * k, dtype=np.float64)
+    lib().oracle_csr_gather(rs.size - 1, _p(rs), _p(ci), k, _p(q), _p(gv))
+    return gv
+
+
+def csr_scatter(rowStarts, colIds, gv, q, k=1):
+    """q with q[col] = gv[row] for every column of every row (ogsOperator.cpp:223-267)."""
+    q = np.array(q, dtype=np.float64)
+    gv = np.ascontiguousarray(gv, dtype=np.float64)
+    rs = np.ascontiguousarray(rowStarts, dtype=np.int32)
+    ci = np.ascontiguousarray(colIds, dtype=np.int32)
+    lib().oracle_csr_scatter(rs.size - 1, _p(rs), _p(ci), k, _p(gv), _p(q))
+    return q
+
+
 STORE_F64, STORE_F32, STORE_BF16 = 0, 1, 2
 
 
@@ -268,6 +293,20 @@ def time_bs8(nx, p, reps, threads):
     t = lib().oracle_time_bs8(nx, p, reps, threads, C.byref(cs), C.byref(st), C.byref(nl), C.byref(ng))
     return {"seconds": t, "checksum": cs.value, "setup_seconds": st.value, "nlocal": nl.value,
             "nglobal": ng.value}
+
+
+def time_ogs(nx, p, reps, threads):
+    """BS6, BS7 and BS8 on one nx^3 hex box at degree p for each thread count in `threads`:
+    {"seconds": {threads: [bs6, bs7, bs8] per call}, "checksums": [bs6, bs7, bs8] of the first call's
+    output, ...}."""
+    thr = np.ascontiguousarray(threads, dtype=np.int32)
+    sec = np.zeros(3 * thr.size)
+    cs = np.zeros(3)
+    st = C.c_double()
+    nl, ng = C.c_int64(), C.c_int64()
+    lib().oracle_time_ogs(nx, p, reps, thr.size, _p(thr), _p(sec), _p(cs), C.byref(st), C.byref(nl), C.byref(ng))
+    return {"seconds": {int(t): sec[3 * i:3 * i + 3].tolist() for i, t in enumerate(thr)},
+            "checksums": cs.tolist(), "setup_seconds": st.value, "nlocal": nl.value, "nglobal": ng.value}
 
 
 def time_stream(which, n, reps, threads):

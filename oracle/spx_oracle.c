@@ -39,6 +39,10 @@
  *                                  KAT, order probe {1e16,1,-1e16} on a corner
  *                                  row, Scatter o Gather == GatherScatter, and
  *                                  agreement with the OKL block-phase form.
+ *   oracle_csr_gather/scatter    : the same row loops over caller CSR arrays;
+ *                                  multiplicity KAT, Sigma invariant, order
+ *                                  probe, scatter closed form (numpy
+ *                                  indexing), Scatter o Gather == csr_gs.
  */
 #include "spx_oracle.h"
 
@@ -692,6 +696,23 @@ void oracle_csr_gs(int32_t nrows, const int32_t *rowStarts, const int32_t *colId
   gs_f64(nrows, rowStarts, colIds, nrows, rowStarts, colIds, ORACLE_OP_ADD, K, q);
 }
 
+/* Gather (Add, double) over a caller-given CSR: gv[row] = 0 + q[c0] + q[c1] ...
+   over the row's columns in stored order, the host loop of
+   ogsOperator.cpp:61-114 (Trans gather: the T-CSR) on arrays not built by
+   oracle_ogs_setup. */
+void oracle_csr_gather(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int K,
+                       const double *q, double *gv) {
+  gather_f64(nrows, rowStarts, colIds, ORACLE_OP_ADD, K, q, gv);
+}
+
+/* Scatter (double) over a caller-given CSR: q[col] = gv[row] for every column
+   of every row, other entries untouched (ogsOperator.cpp:223-267, NoTrans
+   scatter: the T-CSR). */
+void oracle_csr_scatter(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int K,
+                        const double *gv, double *q) {
+  scatter_f64(nrows, rowStarts, colIds, K, gv, q);
+}
+
 /* ogsKernels.okl:32-81 block phases, p_gatherNodesPerBlock = 512, op Add */
 void oracle_ogs_gs_okl(int32_t nblocks, int K, const int32_t *blockStarts,
                        const int32_t *gStarts, const int32_t *gIds,
@@ -820,35 +841,64 @@ static void set_threads(int threads) {
 #endif
 }
 
-/* BS8 on an nx^3 box at degree p: ogs.GatherScatter(q,1,Add,Sym) timed over
-   `reps` calls (bs8Run.cpp:41-49); returns seconds per call. */
-double oracle_time_bs8(int nx, int p, int reps, int threads, double *checksum,
-                       double *setup_seconds, int64_t *nlocal, int64_t *nglobal) {
-  set_threads(threads);
+/* BS6 (ogs.Gather(gq, q, 1, Add, Trans)), BS7 (ogs.Scatter(q, gq, 1,
+   NoTrans)) and BS8 (ogs.GatherScatter(q, 1, Add, Sym)) on one nx^3 hex box
+   at degree p, q = U[-1,1) from seed 0 (bs6Run.cpp:41-50, bs7Run.cpp:41-50,
+   bs8Run.cpp:41-49).  For each of the nthr thread counts threads[t], each
+   kernel runs once untimed and then `reps` timed calls: seconds[3 t + k] is
+   the time per call of BS6, BS7, BS8 (k = 0, 1, 2).  checksums[k] is the
+   long-double sum of the first call's output: gq for BS6, and for BS7 and BS8
+   q after scattering that gq (the same as one gather+scatter of q). */
+void oracle_time_ogs(int nx, int p, int reps, int nthr, const int *threads, double *seconds,
+                     double *checksums, double *setup_seconds, int64_t *nlocal, int64_t *nglobal) {
   const int64_t Nl = (int64_t)nx * nx * nx * (p + 1) * (p + 1) * (p + 1);
   int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * Nl);
+  set_threads(nthr > 0 ? threads[0] : 1);
   double t0 = now_seconds();
   oracle_hex_ids_lex(nx, nx, nx, p, ids);
   oracle_ogs_t *o = oracle_ogs_setup((int32_t)Nl, ids, ORACLE_OGS_SIGNED, 1);
   if (setup_seconds) *setup_seconds = now_seconds() - t0;
+  double *q0 = (double *)malloc(sizeof(double) * Nl);
   double *q = (double *)malloc(sizeof(double) * Nl);
-  oracle_fill_uniform(Nl, q, 0, 0);
-  oracle_ogs_gather_scatter(o, ORACLE_DOUBLE, ORACLE_OP_ADD, ORACLE_SYM, 1, q); /* warm-up */
-  double t1 = now_seconds();
-  for (int r = 0; r < reps; ++r)
-    oracle_ogs_gather_scatter(o, ORACLE_DOUBLE, ORACLE_OP_ADD, ORACLE_SYM, 1, q);
-  double dt = (now_seconds() - t1) / (reps > 0 ? reps : 1);
-  if (checksum) {
-    long double s = 0.0L;
-    for (int64_t i = 0; i < Nl; ++i) s += q[i];
-    *checksum = (double)s;
+  double *gq = (double *)malloc(sizeof(double) * (o->NrowsT > 0 ? o->NrowsT : 1));
+  oracle_fill_uniform(Nl, q0, 0, 0);
+  for (int t = 0; t < nthr; ++t) {
+    set_threads(threads[t]);
+    for (int k = 0; k < 3; ++k) {
+      memcpy(q, q0, sizeof(double) * Nl);
+      if (k == 1) oracle_ogs_gather(o, ORACLE_DOUBLE, ORACLE_OP_ADD, ORACLE_TRANS, 1, q, gq);
+      for (int r = -1; r < reps; ++r) {
+        if (r == 0) t0 = now_seconds();
+        if (k == 0) oracle_ogs_gather(o, ORACLE_DOUBLE, ORACLE_OP_ADD, ORACLE_TRANS, 1, q, gq);
+        else if (k == 1) oracle_ogs_scatter(o, ORACLE_DOUBLE, ORACLE_NOTRANS, 1, gq, q);
+        else oracle_ogs_gather_scatter(o, ORACLE_DOUBLE, ORACLE_OP_ADD, ORACLE_SYM, 1, q);
+        if (r == -1 && t == 0 && checksums) {
+          const double *out = (k == 0) ? gq : q;
+          const int64_t n = (k == 0) ? o->NrowsT : Nl;
+          long double sum = 0.0L;
+          for (int64_t i = 0; i < n; ++i) sum += out[i];
+          checksums[k] = (double)sum;
+        }
+      }
+      seconds[3 * t + k] = (now_seconds() - t0) / (reps > 0 ? reps : 1);
+    }
   }
   if (nlocal) *nlocal = Nl;
   if (nglobal) *nglobal = o->NrowsT;
   oracle_ogs_free(o);
+  free(gq);
   free(q);
+  free(q0);
   free(ids);
-  return dt;
+}
+
+/* BS8 alone: seconds per call; *checksum as oracle_time_ogs's BS8 checksum */
+double oracle_time_bs8(int nx, int p, int reps, int threads, double *checksum,
+                       double *setup_seconds, int64_t *nlocal, int64_t *nglobal) {
+  double sec[3], cs[3];
+  oracle_time_ogs(nx, p, reps, 1, &threads, sec, cs, setup_seconds, nlocal, nglobal);
+  if (checksum) *checksum = cs[2];
+  return sec[2];
 }
 
 /* BS1..BS5 on n doubles, seconds per call.  threads==1 runs the Serial

@@ -79,6 +79,12 @@ void oracle_ogs_gather_scatter(const oracle_ogs_t *o, int type, int op, int tran
 /* Sym Add gatherScatter over caller CSR arrays (ogsOperator.cpp:374-389) */
 void oracle_csr_gs(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int K, double *q);
 
+/* gather (Add) and scatter over caller CSR arrays (ogsOperator.cpp:61-114, 223-267) */
+void oracle_csr_gather(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int K,
+                       const double *q, double *gv);
+void oracle_csr_scatter(int32_t nrows, const int32_t *rowStarts, const int32_t *colIds, int K,
+                        const double *gv, double *q);
+
 /* the OKL block-phase form of gatherScatter (ogsKernels.okl:32-81), Add, K fields */
 void oracle_ogs_gs_okl(int32_t nblocks, int K, const int32_t *blockStarts,
                        const int32_t *gStarts, const int32_t *gIds,
@@ -99,6 +105,8 @@ void oracle_gs_mixed(int storage, int32_t nrows, const int32_t *rowStarts, const
 double oracle_cg_mixed(int storage, int64_t n, double alpha, const void *p, const void *Ap, void *x, void *r);
 
 /* ---- CPU baseline timers (Serial = 1 thread; OpenMP = all host cores) ---- */
+void oracle_time_ogs(int nx, int p, int reps, int nthr, const int *threads, double *seconds,
+                     double *checksums, double *setup_seconds, int64_t *nlocal, int64_t *nglobal);
 double oracle_time_bs8(int nx, int p, int reps, int threads, double *checksum,
                        double *setup_seconds, int64_t *nlocal, int64_t *nglobal);
 double oracle_time_stream(int which, int64_t n, int reps, int threads);

@@ -93,6 +93,16 @@ class DeviceArray:
             check(libspx().spx_memcpy(out.ctypes.data, self.ptr, self.nbytes, 0), "spx_memcpy")
         return out
 
+    def read(self, start, count):
+        """Elements [start, start + count) copied to a new host array."""
+        if not (0 <= start and 0 <= count and start + count <= self.n):
+            raise IndexError(f"range [{start}, {start + count}) outside [0, {self.n}]")
+        out = np.empty(count, dtype=self.dtype)
+        if count:
+            check(libspx().spx_memcpy(out.ctypes.data, self.offset(start), count * self.dtype.itemsize, 0),
+                  "spx_memcpy")
+        return out
+
     def fill_zero(self):
         check(libspx().spx_fill_bytes(self.ptr, self.nbytes, 0), "spx_fill_bytes")
 

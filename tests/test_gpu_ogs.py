@@ -15,7 +15,8 @@ from streamparanumal_amd.device import DeviceArray
 
 pytestmark = pytest.mark.gpu
 
-MESHES = [(1, 1, 1, 1), (2, 1, 1, 1), (2, 2, 2, 2), (3, 2, 1, 3), (5, 3, 2, 4), (4, 4, 4, 15), (8, 8, 8, 7)]
+MESHES = [(1, 1, 1, 1), (2, 1, 1, 1), (2, 2, 2, 2), (3, 2, 1, 3), (5, 3, 2, 4), (4, 4, 4, 15), (8, 8, 8, 7),
+          (10, 2, 2, 7)]  # 71 lattice points along x: the 64-point tile blocks of the hex setup split there
 DTYPES = [np.float64, np.float32, np.int32, np.int64]
 OPS = [spx.ADD, spx.MUL, spx.MAX, spx.MIN]
 TRANS = [spx.SYM, spx.NOTRANS, spx.TRANS]

@@ -307,6 +307,43 @@ def test_order_probe_corner_row():
     assert (1e16 - 1e16) + 1.0 == 1.0  # the other order would give 1
 
 
+@pytest.mark.parametrize("mesh", [(2, 2, 2, 1), (3, 2, 4, 2), (2, 3, 2, 5)])
+def test_csr_gather_scatter_pins(mesh):
+    """The CSR-array gather and scatter: multiplicity KAT, exact sum invariant on integers, the
+    scatter's closed form by numpy indexing (untouched entries included), the composition with the
+    gather+scatter, and the ascending-order probe on a corner row."""
+    ids = o.hex_ids(*mesh)
+    og = o.Ogs(ids)
+    rs, ci = og.rowStartsT, og.colIdsT
+    g1 = o.csr_gather(rs, ci, np.ones(ids.size))
+    counts = {int(k): int(v) for k, v in zip(*np.unique(g1, return_counts=True))}
+    assert counts == multiplicity_counts(*mesh)
+    q = np.floor(o.uniform(ids.size, 6) * 1000)
+    gq = o.csr_gather(rs, ci, q)
+    assert gq.sum() == q.sum()
+    row_of = np.repeat(np.arange(rs.size - 1), np.diff(rs))
+    base = o.uniform(ids.size, 7)
+    keep = ci[rs[1]:rs[2]]                       # row 1 is emptied in the scatter's CSR below
+    rs_cut = np.concatenate([rs[:2], rs[2:] - (rs[2] - rs[1])]).astype(np.int32)
+    ci_cut = np.concatenate([ci[:rs[1]], ci[rs[2]:]]).astype(np.int32)
+    gv = o.uniform(rs.size - 1, 8)
+    out = o.csr_scatter(rs_cut, ci_cut, gv, base)
+    want = base.copy()
+    want[ci] = gv[row_of]
+    want[keep] = base[keep]
+    assert np.array_equal(out, want)
+    qf = o.uniform(ids.size, 9)
+    assert np.array_equal(o.csr_scatter(rs, ci, o.csr_gather(rs, ci, qf), np.zeros(ids.size)), o.csr_gs(rs, ci, qf))
+    assert np.array_equal(o.csr_gather(rs, ci, qf), og.gather(qf, 1, 0, 2))
+    r = int(np.argmax(np.diff(rs) == np.diff(rs).max()))
+    m = ci[rs[r]:rs[r + 1]]
+    probe = np.zeros(ids.size)
+    probe[m[0]], probe[m[1]], probe[m[-1]] = 1e16, -1e16, 1.0
+    assert o.csr_gather(rs, ci, probe)[r] == 1.0     # (1e16 - 1e16) + 1: the big pair cancels first
+    probe[m[1]], probe[m[-1]] = 1.0, -1e16
+    assert o.csr_gather(rs, ci, probe)[r] == 0.0     # (1e16 + 1) rounds to 1e16 before - 1e16
+
+
 def test_okl_block_form_agrees_and_blocks_valid():
     ids = o.hex_ids(8, 8, 8, 7)
     og = o.Ogs(ids)
