@@ -96,6 +96,12 @@ int spxh_ogs_set_gs_format(spxh_ogs_t ogs, int sorted, int xcd_remap);
    its grid points in small 3-D blocks by itself; the gather and scatter keep
    the CSR's order. */
 int spxh_ogs_set_gs_row_keys(spxh_ogs_t ogs, const int64_t *keys);
+/* As spxh_ogs_set_gs_row_keys, and no tile holds rows of two different keys
+   (rows of one key are cut into tiles by capacity only): each key names a
+   group of rows meant to share tiles, e.g. the rows whose members fill whole
+   memory lines of the vector.  No reference counterpart; results are
+   identical. */
+int spxh_ogs_set_gs_tiles(spxh_ogs_t ogs, const int64_t *keys);
 /* layouts of the T-CSR gather (Trans) and scatter (NoTrans): 1 = the
    sorted-run blocks of the gather+scatter, 0 = T-CSR row blocks, -1 = keep;
    both sorted by default.  Results are identical. */

@@ -128,6 +128,10 @@ class ogsOperator_t {
   // layout as gsSrb when gsRowOrder is empty).
   SortedRunBlocks gsSrb, rowSrb;
   memory<dlong> gsRowOrder;
+  // gsTileCut[i] = 1: a gather+scatter tile starts at row gsRowOrder[i]
+  // (empty: tiles are cut only by capacity).  A hex box's setup cuts tiles so
+  // that each memory line of the vector is written by one tile.
+  memory<uint8_t> gsTileCut;
   bool gsSorted = true;
   // The T-CSR gather (Trans) and scatter (NoTrans) on the same layout or on
   // the T-CSR row blocks of blockNnz entries.  Measured on MI355X at 64^3 hex
@@ -175,7 +179,8 @@ class ogsOperator_t {
   // Rows of the gather+scatter tiles in this order (a permutation of the
   // T-CSR rows; empty: the CSR's); results do not change.  validate: check
   // that it is a permutation (callers that built one may skip it).
-  void SetGsRowOrder(const memory<dlong> &order, bool validate = true);
+  // cuts: optional tile starts, one flag per entry of order (see gsTileCut).
+  void SetGsRowOrder(const memory<dlong> &order, bool validate = true, const memory<uint8_t> &cuts = memory<uint8_t>());
   // gsSrb from rowSrb and gsRowOrder, with these launch hints
   void BuildGsSrb(int xcdRemap, int tileLanes);
 
@@ -229,28 +234,32 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
 // ~1e8-node boxes at p = 1..15).
 void HexGsBlockShape(const int p, int &bx, int &by, int &bz);
 
-// Parallel exclusive prefix sum of v[0..n) in place; returns the total.
+// Parallel exclusive prefix sum of v[0..n) in place; returns the total.  The
+// slices follow the team the runtime actually gives, which may be smaller than
+// asked for (OMP_THREAD_LIMIT, OMP_DYNAMIC, nesting).
 template <typename V>
 inline V ExclusiveScan(V *v, dlong n) {
   if (n <= 0) return 0;
-  int nt = 1;
+  int nt = 1, team = 1;
 #ifdef _OPENMP
   nt = omp_get_max_threads();
 #endif
   std::vector<V> part(nt + 1, 0);
 #pragma omp parallel num_threads(nt)
   {
-    int t = 0;
+    int t = 0, T = 1;
 #ifdef _OPENMP
     t = omp_get_thread_num();
+    T = omp_get_num_threads();
 #endif
-    const dlong lo = (dlong)((int64_t)n * t / nt), hi = (dlong)((int64_t)n * (t + 1) / nt);
+    if (t == 0) team = T;
+    const dlong lo = (dlong)((int64_t)n * t / T), hi = (dlong)((int64_t)n * (t + 1) / T);
     V sum = 0;
     for (dlong i = lo; i < hi; ++i) sum += v[i];
     part[t + 1] = sum;
 #pragma omp barrier
 #pragma omp single
-    for (int k = 0; k < nt; ++k) part[k + 1] += part[k];
+    for (int k = 0; k < T; ++k) part[k + 1] += part[k];
     V run = part[t];
     for (dlong i = lo; i < hi; ++i) {
       const V x = v[i];
@@ -258,13 +267,14 @@ inline V ExclusiveScan(V *v, dlong n) {
       run += x;
     }
   }
-  return part[nt];
+  return part[team];
 }
 
 // Greedy row blocking: consecutive rows, at most `cap` entries per block.
-// Aborts when one row alone exceeds cap (ogsOperator.cpp:535-536).
+// Aborts when one row alone exceeds cap (ogsOperator.cpp:535-536).  With
+// `cuts` (one flag per row), a block also starts at every flagged row.
 void blockRowsGreedy(const dlong Nrows, const memory<dlong> rowStarts, const int cap, dlong &Nblocks,
-                     memory<dlong> &blockStarts);
+                     memory<dlong> &blockStarts, const uint8_t *cuts = nullptr);
 
 }  // namespace ogs
 }  // namespace libp

@@ -240,6 +240,29 @@ int spxh_ogs_set_gs_row_keys(spxh_ogs_t h, const int64_t *keys) {
   });
 }
 
+int spxh_ogs_set_gs_tiles(spxh_ogs_t h, const int64_t *keys) {
+  if (!h || !keys) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_tiles: null");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    const dlong n = op->NrowsT;
+    if (n == 0) {
+      op->SetGsRowOrder(memory<dlong>());
+      return;
+    }
+    std::vector<std::pair<int64_t, dlong>> kr(n);
+    for (dlong r = 0; r < n; ++r) kr[r] = {keys[r], r};
+    std::sort(kr.begin(), kr.end());
+    memory<dlong> order(n);
+    memory<uint8_t> cuts(n);
+    for (dlong i = 0; i < n; ++i) {
+      order[i] = kr[i].second;
+      cuts[i] = (i == 0) || kr[i].first != kr[i - 1].first;
+    }
+    op->SetGsRowOrder(order, false, cuts);
+  });
+}
+
 int spxh_ogs_autotune(spxh_ogs_t h, int type, int storage, int k, int reps, spxh_gs_tuning_t *best) {
   if (!h || !best) return err(SPX_ERR_ARG, "spxh_ogs_autotune: null");
   if (storage < -1 || storage > SPX_STORE_BF16) return err(SPX_ERR_ARG, "spxh_ogs_autotune: storage in -1..2");

@@ -91,25 +91,28 @@ LocalCSR BuildHexBoxCSR(const dlong nx, const dlong ny, const dlong nz, const in
       for (int64_t I = 0; I < H.GX; ++I)
         rowStartsT[H.row(I, J, K)] = (dlong)(H.mult(I, H.GX) * H.mult(J, H.GY) * H.mult(K, H.GZ));
   {
-    // exclusive scan (entries are 1..8, the total is Nl)
-    int nt = 1;
+    // exclusive scan (entries are 1..8, the total is Nl); the shares follow
+    // the team the runtime actually gives, which may be smaller than asked for
+    int nt = 1, team = 1;
 #ifdef _OPENMP
     nt = omp_get_max_threads();
 #endif
     std::vector<int64_t> part(nt + 1, 0);
 #pragma omp parallel num_threads(nt)
     {
-      int t = 0;
+      int t = 0, T = 1;
 #ifdef _OPENMP
       t = omp_get_thread_num();
+      T = omp_get_num_threads();
 #endif
-      const int64_t lo = Ng * t / nt, hi = Ng * (t + 1) / nt;
+      if (t == 0) team = T;
+      const int64_t lo = Ng * t / T, hi = Ng * (t + 1) / T;
       int64_t s = 0;
       for (int64_t r = lo; r < hi; ++r) s += rowStartsT[r];
       part[t + 1] = s;
 #pragma omp barrier
 #pragma omp single
-      for (int k = 0; k < nt; ++k) part[k + 1] += part[k];
+      for (int k = 0; k < T; ++k) part[k + 1] += part[k];
       int64_t run = part[t];
       for (int64_t r = lo; r < hi; ++r) {
         const int64_t m = rowStartsT[r];
@@ -117,7 +120,7 @@ LocalCSR BuildHexBoxCSR(const dlong nx, const dlong ny, const dlong nz, const in
         run += m;
       }
     }
-    rowStartsT[Ng] = (dlong)part[nt];
+    rowStartsT[Ng] = (dlong)part[team];
   }
   // members: node l of element e lands at its element's rank among the elements sharing its point
 #pragma omp parallel for collapse(3) schedule(static)
@@ -148,11 +151,12 @@ LocalCSR BuildHexBoxCSR(const dlong nx, const dlong ny, const dlong nz, const in
 #endif
 #pragma omp parallel num_threads(nt)
     {
-      int t = 0;
+      int t = 0, T = 1;
 #ifdef _OPENMP
       t = omp_get_thread_num();
+      T = omp_get_num_threads();
 #endif
-      const int64_t lo = Ng * t / nt, hi = Ng * (t + 1) / nt;
+      const int64_t lo = Ng * t / T, hi = Ng * (t + 1) / T;
       lcgRange((uint64_t)rank, lo, hi - lo, chosen.data() + lo);
     }
   }

@@ -12,7 +12,7 @@ namespace libp {
 namespace ogs {
 
 void blockRowsGreedy(const dlong Nrows, const memory<dlong> rowStarts, const int cap, dlong &Nblocks,
-                     memory<dlong> &blockStarts) {
+                     memory<dlong> &blockStarts, const uint8_t *cuts) {
   Nblocks = 0;
   std::vector<dlong> starts;
   starts.push_back(0);
@@ -21,7 +21,7 @@ void blockRowsGreedy(const dlong Nrows, const memory<dlong> rowStarts, const int
     const dlong sz = rowStarts[r + 1] - rowStarts[r];
     LIBP_ABORT("Multiplicity of a global node (" << sz << ") exceeds the row block capacity " << cap,
                sz > cap);
-    if (acc + sz > cap && acc > 0) {
+    if ((acc + sz > cap || (cuts && cuts[r])) && acc > 0) {
       starts.push_back(r);
       acc = 0;
     }
@@ -112,7 +112,7 @@ void ogsOperator_t::BuildGsSrb(int xcd, int lanes) {
     }
     dlong nb = 0;
     memory<dlong> bs;
-    blockRowsGreedy(NrowsT, rs, gsBlockNnz, nb, bs);
+    blockRowsGreedy(NrowsT, rs, gsBlockNnz, nb, bs, gsTileCut.length() ? gsTileCut.ptr() : nullptr);
     gsSrb = BuildSortedRunBlocks(NrowsT, rs, ci, nb, bs, gsBlockNnz, flags);
     gsSrb.Upload(platform);
   }
@@ -120,7 +120,7 @@ void ogsOperator_t::BuildGsSrb(int xcd, int lanes) {
   gsSrb.tileLanes = lanes;
 }
 
-void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order, const bool validate) {
+void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order, const bool validate, const memory<uint8_t> &cuts) {
   if (order.length() && validate) {
     LIBP_ABORT("gather+scatter row order must list the " << NrowsT << " rows", (dlong)order.length() != NrowsT);
     std::vector<char> seen(NrowsT, 0);
@@ -130,7 +130,10 @@ void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order, const bool validat
       seen[r] = 1;
     }
   }
+  LIBP_ABORT("gather+scatter tile cuts need a row order with one flag per row",
+             cuts.length() && (!order.length() || cuts.length() != order.length()));
   gsRowOrder = order;
+  gsTileCut = cuts;
   BuildGsSrb(gsSrb.xcdRemap, gsSrb.tileLanes);
 }
 
@@ -151,12 +154,14 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
   }
   // stable counting sort by block: each thread counts its contiguous share of
   // the rows, the counts are scanned block-major then thread-major, and each
-  // thread places its rows in row order
-  const int T = std::max(1, std::min(omp_get_max_threads(), (int)((Nrows + 65535) / 65536)));
-  std::vector<int64_t> count((size_t)T * nkeys, 0);
-#pragma omp parallel num_threads(T)
+  // thread places its rows in row order.  The shares follow the team the
+  // runtime actually gives (it may be smaller than asked for: OMP_THREAD_LIMIT,
+  // OMP_DYNAMIC, nesting).
+  const int Tmax = std::max(1, std::min(omp_get_max_threads(), (int)((Nrows + 65535) / 65536)));
+  std::vector<int64_t> count((size_t)Tmax * nkeys, 0);
+#pragma omp parallel num_threads(Tmax)
   {
-    const int t = omp_get_thread_num();
+    const int T = omp_get_num_threads(), t = omp_get_thread_num();
     const dlong r0 = (dlong)((int64_t)Nrows * t / T), r1 = (dlong)((int64_t)Nrows * (t + 1) / T);
     int64_t *c = count.data() + (size_t)t * nkeys;
     for (dlong r = r0; r < r1; ++r) ++c[key[r]];
@@ -179,13 +184,13 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
 
 void HexGsBlockShape(const int p, int &bx, int &by, int &bz) {
   // Measured on MI355X, us per call (CSR order -> blocks): ~1e8 nodes, FP64:
-  // p=1 640 -> 436 (8x4x4), p=2 601 -> 447 (8x8x4), p=3 387 -> 361 and
-  // p=4 415 -> 410 (16x8x2), p=11 383 -> 342 and p=15 357 -> 303 (64x4x1);
-  // 64^3 p=7, FP64 / FP32 / BF16 storage (medians of 3 interleaved rounds):
-  // 469 / 347 / 370 -> 453 / 341 / 368 (64x8x1), where 64x4x1 gives
-  // 460 / 335 / 357 and 16x8x2 460 / 366 / 393.  From p=5 up the shapes
-  // tried (8x8x4 to 128x4x1) differ by at most 3 %; p=4 8x8x4 398 against
-  // 410 for 16x8x2.
+  // p=1 640 -> 436 (8x4x4), p=2 601 -> 447 (8x8x4), p=3 387 -> 361
+  // (16x8x2), p=4 415 -> 398 (8x8x4; 410 with 16x8x2); 64^3 p=7, FP64 /
+  // FP32 / BF16 storage (medians of 3 interleaved rounds): 469 / 347 / 370
+  // -> 453 / 341 / 368 with 64x8x1, where 64x4x1 gives 460 / 335 / 357 and
+  // 16x8x2 460 / 366 / 393.  From p=5 up the shapes tried (8x8x4 to
+  // 128x4x1) differ by at most 3 %, so every p >= 6 takes 64x8x1 (p=15
+  // 357 -> 304).
   if (p <= 1) {
     bx = 8; by = 4; bz = 4;
   } else if (p == 2 || p == 4) {

@@ -368,20 +368,29 @@ void ogsBase_t::SetupBatch(const std::vector<const ogsBase_t *> &parts, platform
   NgatherGlobal = Ngather;
   gatherLocal = std::make_shared<ogsOperator_t>(platform, kind == Unsigned ? Unsigned : Signed, NlocalP, NlocalT, N,
                                                 rsT, ciT, rsN, ciN);
-  // the parts' gather+scatter row orders, side by side
-  bool ordered = false;
-  for (const ogsBase_t *p : parts) ordered = ordered || p->gatherLocal->gsRowOrder.length() > 0;
+  // the parts' gather+scatter row orders and tile cuts, side by side
+  bool ordered = false, cut = false;
+  for (const ogsBase_t *p : parts) {
+    ordered = ordered || p->gatherLocal->gsRowOrder.length() > 0;
+    cut = cut || p->gatherLocal->gsTileCut.length() > 0;
+  }
   if (ordered) {
     memory<dlong> order(NlocalT);
+    memory<uint8_t> cuts;
+    if (cut) cuts.malloc(NlocalT > 0 ? NlocalT : 1);
     dlong off = 0;
     for (const ogsBase_t *p : parts) {
       const ogsOperator_t &op = *p->gatherLocal;
-      const bool has = op.gsRowOrder.length() > 0;
+      const bool has = op.gsRowOrder.length() > 0, hasCut = op.gsTileCut.length() > 0;
 #pragma omp parallel for schedule(static)
-      for (dlong r = 0; r < op.NrowsT; ++r) order[off + r] = off + (has ? op.gsRowOrder[r] : r);
+      for (dlong r = 0; r < op.NrowsT; ++r) {
+        order[off + r] = off + (has ? op.gsRowOrder[r] : r);
+        if (cut) cuts[off + r] = (r == 0) || (hasCut && op.gsTileCut[r]);
+      }
       off += op.NrowsT;
     }
-    gatherLocal->SetGsRowOrder(order, false);
+    if (cut && NlocalT == 0) cuts = memory<uint8_t>();
+    gatherLocal->SetGsRowOrder(order, false, cuts);
   }
   setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

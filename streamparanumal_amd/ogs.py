@@ -289,6 +289,14 @@ class Ogs:
             raise ValueError(f"need {self.NlocalT} keys, got {k.size}")
         checkh(libspxhost().spxh_ogs_set_gs_row_keys(self.h, k.ctypes.data), "spxh_ogs_set_gs_row_keys")
 
+    def set_gs_tiles(self, keys):
+        """Rows of the sorted-run gather+scatter tiles sorted by keys, and no tile holding rows of two
+        keys.  Results do not change."""
+        k = np.ascontiguousarray(keys, dtype=np.int64)
+        if k.size != self.NlocalT:
+            raise ValueError(f"need {self.NlocalT} keys, got {k.size}")
+        checkh(libspxhost().spxh_ogs_set_gs_tiles(self.h, k.ctypes.data), "spxh_ogs_set_gs_tiles")
+
     def set_gather_format(self, gather_sorted=None, scatter_sorted=None):
         """T-CSR gather (Trans) and scatter (NoTrans) on the sorted-run blocks (True) or the T-CSR row
         blocks (False); None keeps the current choice (both sorted by default)."""

@@ -283,3 +283,22 @@ def test_hex_box_setup_rejects():
         ogs.build_csr_hex(1, 1, 1, 0)
     with pytest.raises(SpxError):
         ogs.build_csr_hex(2, 2, 2, 3, spx.UNSIGNED, True)    # Unsigned cannot be unique (ogsSetup.cpp:92)
+
+
+def test_hex_box_setup_with_a_smaller_thread_team():
+    """OMP_THREAD_LIMIT below the requested team: the closed-form setup's scans split the rows by
+    the team the runtime gives, so the operator is still the generic setup's."""
+    import os
+    import subprocess
+    import sys
+    code = ("import numpy as np, streamparanumal_amd as spx; from streamparanumal_amd import ogs\n"
+            "box = (6, 5, 4, 3)\n"
+            "i0, c0 = ogs.build_csr(ogs.hex_ids(*box), spx.SIGNED, True)\n"
+            "i1, c1 = ogs.build_csr_hex(*box, spx.SIGNED, True)\n"
+            "assert np.array_equal(i0, i1)\n"
+            "assert all(np.array_equal(c0[k], c1[k]) if isinstance(c0[k], np.ndarray) else c0[k] == c1[k] for k in c0)\n"
+            "print('ok')\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, OMP_NUM_THREADS="8", OMP_THREAD_LIMIT="3", PYTHONPATH=root)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr[-2000:]
