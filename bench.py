@@ -17,10 +17,14 @@ the oracle's restatement of the reference's Serial mode on a bounded sample.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
+import glob
+import hashlib
 import json
 import math
 import os
+import platform as pyplatform
 import re
+import subprocess
 import sys
 import time
 
@@ -30,6 +34,43 @@ sys.path.insert(0, ROOT)
 SPEC_HBM_GBS = 8000.0
 METRIC = "HBM GB/s (% peak) + GDOF/s per BS kernel; mixed/batched vs plain FP64"
 PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "bs8_gather_scatter_64cube_p7_counters.md")
+BIN = os.path.join(ROOT, "streamparanumal_amd", "bin")
+
+# kernel-name patterns of the suite's dominant kernels, for the counter passes
+KERNEL_PATTERNS = {
+    "BS1_copy": r"k_copy2<4, true>",
+    "BS2_axpy": r"k_axpy2<",
+    "BS3_norm_reference": r"k_red_ref1<0,",
+    "BS4_dot_reference": r"k_red_ref1<1,",
+    "BS5_cg_update_reference": r"k_red_ref1<2,",
+    "BS5_cg_full_reference": r"k_red_ref1<3,",
+    "BS3_norm_fast": r"k_red_fast1<0,",
+    "BS4_dot_fast": r"k_red_fast1<1,",
+    "BS5_cg_update_fast": r"k_red_fast1<2,",
+    "BS5_cg_full_fast": r"k_red_fast1<3,",
+    "BS5_cg_full_f32": r"k_cg_mixed1<float",
+    "BS5_cg_full_bf16": r"k_cg_mixed1<unsigned short",
+    "BS6_gather": r"k_gather_srb<double",
+    "BS7_scatter": r"k_scatter_srb<double",
+    "BS8_gather_scatter": r"k_gs_srb<double",
+    "BS8_gather_scatter_f32": r"k_gs_srb<float",
+    "BS8_gather_scatter_bf16": r"k_gs_srb<unsigned short",
+}
+
+
+def source_hash():
+    """16 hex digits of sha256 over the sources that decide what the kernels do and move: the HIP
+    kernels, the host layer that builds their layouts, and the C headers."""
+    pkg = os.path.join(ROOT, "streamparanumal_amd")
+    files = sorted(glob.glob(os.path.join(pkg, "csrc", "*")) + glob.glob(os.path.join(pkg, "libp", "src", "*.cpp")) +
+                   glob.glob(os.path.join(pkg, "libp", "include", "**", "*.hpp"), recursive=True) +
+                   glob.glob(os.path.join(ROOT, "include", "*.h")))
+    h = hashlib.sha256()
+    for f in files:
+        h.update(os.path.relpath(f, ROOT).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def log(*a):
@@ -66,12 +107,185 @@ def timed(fn, steps, warmup, settle=SETTLE_SECONDS):
 
 
 def read_traffic():
-    """HBM bytes per BS8 launch from the committed rocprofv3 counter summary, if present."""
+    """(HBM bytes per BS8 launch, provenance) from the committed rocprofv3 counter summary, or
+    (None, reason) when it is missing or was made from other kernel sources than these."""
     if not os.path.exists(PROFILE_COUNTERS):
-        return None
+        return None, "no committed counter profile"
     text = open(PROFILE_COUNTERS).read()
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
-    return float(m.group(1)) if m else None
+    h = re.search(r"source_sha16:\s*([0-9a-f]{16})", text)
+    if not m:
+        return None, "no traffic figure in the committed profile"
+    if not h or h.group(1) != source_hash():
+        return None, "committed profile is from other kernel sources (stale): not used"
+    return float(m.group(1)), f"profiles/{os.path.basename(PROFILE_COUNTERS)} (same kernel sources)"
+
+
+def traffic_probe(nx, p, log2n, reps=3):
+    """The launches the counter passes measure (run as the profiled child process): the plain
+    calibration copy of 2^27 doubles, then `reps` calls of every suite kernel on the bench's inputs."""
+    import numpy as np
+
+    import streamparanumal_amd as spx
+    from streamparanumal_amd import device, ogs, streaming
+    from streamparanumal_amd.device import DeviceArray
+
+    ogs.platform_open(0)
+    n = 1 << log2n
+    a, b, pv, xv = (DeviceArray(n) for _ in range(4))
+    for i, v in enumerate((a, b, pv, xv)):
+        device.fill_uniform(v, 1, i * n)
+    c0, c1 = DeviceArray(1 << 27), DeviceArray(1 << 27)
+    device.fill_uniform(c0, 1, 0)
+    device.set_tuning("bs1", unroll=4, nontemporal=0)
+    for _ in range(reps):
+        streaming.copy(c0, c1)                       # k_copy2<4, false>: 1 GiB each way
+    device.set_tuning("bs1")
+    del c0, c1
+    for _ in range(reps):
+        streaming.copy(a, b)
+    for _ in range(reps):
+        streaming.axpy(0.37, a, -1.21, b)
+    device.fill_uniform(b, 1, n)
+    for mode in (spx.REDUCE_REFERENCE, spx.REDUCE_FAST):
+        red = streaming.Reducer(n, mode, 256)
+        for _ in range(reps):
+            red.norm(a)
+        for _ in range(reps):
+            red.dot(a, b)
+        for _ in range(reps):
+            red.cg_update(a, 1e-9, b)
+        for _ in range(reps):
+            red.cg_full(1e-9, pv, a, xv, b)
+    mix = streaming.MixedCG()
+    for dt in (np.float32, np.uint16):
+        vs = [DeviceArray(n, dt) for _ in range(4)]
+        for v, src in zip(vs, (pv, a, xv, b)):
+            device.convert(src, v)
+        for _ in range(reps):
+            mix.cg_full(1e-9, *vs)
+        del vs
+    del a, b, pv, xv
+    g = ogs.Ogs.hex(nx, nx, nx, p, keep_ids=False)
+    q, gq = DeviceArray(g.N), DeviceArray(g.NgatherGlobal)
+    device.fill_uniform(q, 1, 0)
+    for _ in range(reps):
+        g.gather_scatter(q)
+    for _ in range(reps):
+        g.gather(gq, q, 1, spx.ADD, spx.TRANS)
+    for _ in range(reps):
+        g.scatter(q, gq, 1, spx.NOTRANS)
+    for dt in (np.float32, np.uint16):
+        qs = DeviceArray(g.N, dt)
+        device.convert(q, qs)
+        for _ in range(reps):
+            g.gather_scatter_mixed(qs)
+        del qs
+    device.finish()
+    device.check_kernel_errors()
+
+
+def counter_suite(args):
+    """FETCH_SIZE / WRITE_SIZE of every suite kernel, measured in this run by two rocprofv3 passes
+    over `bench.py --traffic-probe`: {entry: {"read", "write", "traffic", "dispatches"}}."""
+    from streamparanumal_amd import counters
+    if counters.rocprofv3() is None:
+        return None, "rocprofv3 not found"
+    cmd = [sys.executable, os.path.abspath(__file__), "--traffic-probe", "--nx", str(args.nx), "--degree",
+           str(args.degree), "--log2n", str(args.log2n)]
+    env = dict(os.environ, PYTHONPATH=ROOT + os.pathsep + os.environ.get("PYTHONPATH", ""))
+    try:
+        out = counters.measure(cmd, KERNEL_PATTERNS, timeout=240, env=env)
+    except Exception as e:  # the roofline then falls back to the committed profile
+        return None, f"counter passes failed: {str(e)[-400:]}"
+    return out, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes in this run"
+
+
+def run_driver(name, *args, timeout=300):
+    """Run bin/<name> with `args` and return its printed rows, {row name: [numbers]}."""
+    r = subprocess.run([os.path.join(BIN, name), *args], capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"{name} {' '.join(args)} failed ({r.returncode}): {(r.stderr or r.stdout)[-800:]}")
+    rows = {}
+    for ln in r.stdout.splitlines():
+        m = re.match(r"\s*(\w+) = \[([^\]]*)\]", ln)
+        if m:
+            rows[m.group(1)] = [float(x) for x in re.split(r"[,;\s]+", m.group(2).strip()) if x]
+    return rows
+
+
+def driver_suite():
+    """Rows of the C++ drivers: BS0 as 10^4 empty launches, plain and replayed from a graph, timed on
+    the host after a finish as bs0Run.cpp:70-77 does; BS6-BS8 at 8^3 p=7 (seed 0) and 64^3 p=7
+    (seed 1) with three timings per call: back-to-back calls (the BSn row), kernel-only events, and
+    a host finish after every call, which is what the reference's ogs_t does inside each call."""
+    out = {}
+    plain = run_driver("BS0", "-n", "10000")["BS0"]
+    graph = run_driver("BS0", "-n", "10000", "--graph", "TRUE")["BS0"]
+    out["BS0_launch"] = {"launches": int(plain[0]), "plain_us_per_launch": round(plain[1] * 1e6, 3),
+                         "graph_us_per_launch": round(graph[1] * 1e6, 3), "timer": "C++ driver bin/BS0"}
+    for tag, nx, seed in (("8^3 p=7", 8, 0), ("64^3 p=7", 64, 1)):
+        for n, what in (("BS6", "gather"), ("BS7", "scatter"), ("BS8", "gather_scatter")):
+            rows = run_driver(n, "-nx", str(nx), "-ny", str(nx), "-nz", str(nx), "-p", "7", "--seed", str(seed))
+            b2b, dev, syn = rows[n], rows[n + "device"], rows[n + "sync"]
+            dofs = b2b[1]
+            out[f"{n}_{what}_{nx}cube_p7"] = {
+                "config": tag, "dofs": int(dofs),
+                "back_to_back_us": round(b2b[2] * 1e6, 2), "back_to_back_GBs": b2b[4],
+                "kernel_us": round(dev[2] * 1e6, 2), "kernel_GBs": dev[4],
+                "synced_us": round(syn[2] * 1e6, 2), "synced_GBs": syn[4], "synced_GDOFs": round(syn[3] / 1e9, 3)}
+            log("driver", n, tag, json.dumps(out[f"{n}_{what}_{nx}cube_p7"]))
+    return out
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return pyplatform.processor() or "unknown"
+
+
+def cpu_rows(c1_gpu_checksum=None):
+    """The reference's Serial and OpenMP modes restated by the oracle, timed on this host:
+    BASELINE config 1 (BS8 on 8^3 p=7, seed 0, Serial) with the checksum of its output; BS1-BS5
+    Serial on N = 2^24 and OpenMP on N = 2^27; BS6-BS8 Serial and OpenMP on a 32^3 p=7 box."""
+    from oracle import pyoracle
+    pyoracle.lib()
+    T = pyoracle.max_threads()
+    rows = {"cpu_model": cpu_model(), "threads_available": T}
+    c1 = pyoracle.time_ogs(8, 7, 200, [1])
+    nl, ng = c1["nlocal"], c1["nglobal"]
+    t = c1["seconds"][1][2]
+    rows["C1_BS8_serial"] = {"ms_per_call": round(t * 1e3, 4), "GBs": round((4 * (ng + 1) + 20 * nl) / t / 1e9, 3),
+                             "GDOFs": round(ng / t / 1e9, 4), "checksum": c1["checksums"][2],
+                             "gpu_checksum": c1_gpu_checksum, "cores": 1,
+                             "sample": "8^3 hex p=7, U[-1,1) seed 0, 200 calls; checksum = long-double sum of one "
+                                       "gather+scatter's output"}
+    bytes_per = {1: 16, 2: 24, 3: 8, 4: 16, 5: 24}
+    for threads, log2n, reps in ((1, 24, 3), (T, 27, 5)):
+        n = 1 << log2n
+        mode = "serial" if threads == 1 else "openmp"
+        for w, name in ((1, "BS1_copy"), (2, "BS2_axpy"), (3, "BS3_norm"), (4, "BS4_dot"), (5, "BS5_cg_update")):
+            sec = pyoracle.time_stream(w, n, reps, threads)
+            rows[f"{name}_{mode}"] = {"N": n, "ms_per_call": round(sec * 1e3, 3),
+                                      "GBs": round(bytes_per[w] * n / sec / 1e9, 2), "cores": threads}
+    r = pyoracle.time_ogs(32, 7, 3, [1, T])
+    nl, ng = r["nlocal"], r["nglobal"]
+    b = {"BS6_gather": 4 * (ng + 1) + 12 * nl + 8 * ng, "BS7_scatter": 4 * (ng + 1) + 12 * nl + 8 * ng,
+         "BS8_gather_scatter": 4 * (ng + 1) + 20 * nl}
+    for threads in (1, T):
+        mode = "serial" if threads == 1 else "openmp"
+        for k, name in enumerate(("BS6_gather", "BS7_scatter", "BS8_gather_scatter")):
+            sec = r["seconds"][threads][k]
+            rows[f"{name}_{mode}"] = {"box": "32^3 p=7", "ms_per_call": round(sec * 1e3, 3),
+                                      "GBs": round(b[name] / sec / 1e9, 2), "GDOFs": round(ng / sec / 1e9, 4),
+                                      "cores": threads}
+    for k, v in rows.items():
+        log("cpu", k, json.dumps(v))
+    return rows
 
 
 def cpu_baseline():
@@ -320,10 +534,15 @@ def degree_sweep(K, W, seed=2, target=1e8):
     return rows
 
 
-def report(args, K, W, props, nx, p, Nl, Ng, n, bytes_bs8, host8, ev8, measured_peak, cpu, suite):
-    """The one JSON line: BS8 value and roofline, the suite and the CPU baseline."""
+def report(args, K, W, props, nx, p, Nl, Ng, n, bytes_bs8, host8, ev8, measured_peak, cpu, suite, traffic=None,
+           traffic_source=None):
+    """The one JSON line: BS8 value and roofline, the suite and the CPU baseline.  `traffic` is the BS8
+    kernel's HBM bytes per launch from counters measured in this run; without it the committed
+    profile's figure is used when it was made from the same kernel sources, else null."""
     value = bytes_bs8 / host8 / 1e9
     achieved = bytes_bs8 / ev8 / 1e9
+    if traffic is None:
+        traffic, traffic_source = read_traffic()
     workload = f"BS8 gather+scatter FP64 (Add, Sym, K=1) on a {nx}^3 hex box at p={p} ({Nl} local / {Ng} global nodes)"
     if n is not None:
         workload += f"; suite BS1-BS5 at N=2^{args.log2n}"
@@ -346,9 +565,11 @@ def report(args, K, W, props, nx, p, Nl, Ng, n, bytes_bs8, host8, ev8, measured_
                    "pct_spec_peak": round(100 * value / SPEC_HBM_GBS, 1),
                    "pct_measured_peak": round(100 * achieved / measured_peak, 1) if measured_peak else None,
                    "measured_peak_GBs": round(measured_peak, 1) if measured_peak else None,
+                   "traffic_source": traffic_source, "source_sha16": source_hash(),
                    "device": props["name"] or f"{props['arch']} ({props['compute_units']} CUs)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": SPEC_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / SPEC_HBM_GBS, 4), "traffic": read_traffic()},
+                     "frac": round(achieved / SPEC_HBM_GBS, 4),
+                     "traffic": round(traffic) if traffic is not None else None},
         "cpu_baseline": cpu,
         "suite": suite,
     }
@@ -374,7 +595,15 @@ def main():
     ap.add_argument("--only-bs8", action="store_true",
                     help="time the BS8 step alone (no suite, no CPU baseline): the command the committed "
                          "rocprofv3 summaries of the BS8 kernel are taken from")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure every kernel's HBM traffic")
+    ap.add_argument("--no-drivers", action="store_true", help="skip the C++ driver rows (BS0, BS6-BS8 per call)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="internal: the launches the counter passes profile (run by bench.py itself)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args.nx, args.degree, args.log2n)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
@@ -475,7 +704,20 @@ def main():
     device.fill_uniform(b, 1, n)
     h, e = timed(lambda: streaming.copy(a, b), K, W)
     entry("BS1_copy", 16 * n, n, h, e)
-    measured_peak = 16 * n / e / 1e9
+    # measured peak: the fastest copy variant of this run (streaming or plain accesses, 2-8 pairs per lane)
+    variants = {}
+    for nt in (1, 0):
+        for unroll in (2, 4, 8):
+            device.set_tuning("bs1", unroll=unroll, nontemporal=nt)
+            _, ev = timed(lambda: streaming.copy(a, b), K, W, settle=0.1)
+            variants[f"{'nontemporal' if nt else 'plain'}_u{unroll}"] = round(16 * n / ev / 1e9, 1)
+    device.set_tuning("bs1")
+    best = max(variants, key=variants.get)
+    measured_peak = max(variants[best], 16 * n / e / 1e9)
+    suite["BS1_copy"]["variants_GBs"] = variants
+    suite["measured_peak"] = {"GBs": round(measured_peak, 1), "variant": best if variants[best] >= 16 * n / e / 1e9
+                              else "library default", "pct_spec": round(100 * measured_peak / SPEC_HBM_GBS, 1)}
+    log("measured peak", json.dumps(suite["measured_peak"]))
     h, e = timed(lambda: streaming.axpy(0.37, a, -1.21, b), K, W)
     entry("BS2_axpy", 24 * n, n, h, e)
     device.fill_uniform(b, 1, n)  # repeated axpy with |beta| > 1 grows b without bound: fresh inputs
@@ -525,29 +767,22 @@ def main():
         del vs, x0, r0
     del pvec, xvec, a, b, r64
     for name, d in suite.items():
-        d["pct_measured_peak"] = round(100 * d["GBs"] / measured_peak, 1)
+        if "GBs" in d and "bytes_per_call" in d:
+            d["pct_measured_peak"] = round(100 * d["GBs"] / measured_peak, 1)
 
-    # ---------------------------------------------------------------- BS0
-    ta, tb = DeviceArray(1), DeviceArray(1)
-    nl = 10000
-    for _ in range(100):
-        streaming.bs0(1, ta, tb)
-    device.finish()
-    t0 = time.perf_counter()
-    for _ in range(nl):
-        streaming.bs0(1, ta, tb)
-    device.finish()
-    plain = (time.perf_counter() - t0) / nl
-    graph = device.Graph.capture(lambda: [streaming.bs0(1, ta, tb) for _ in range(nl)])
-    graph.launch()
-    device.finish()
-    t0 = time.perf_counter()
-    graph.launch()
-    device.finish()
-    replay = (time.perf_counter() - t0) / nl
-    suite["BS0_launch"] = {"launches": nl, "plain_us_per_launch": round(plain * 1e6, 3),
-                           "graph_us_per_launch": round(replay * 1e6, 3)}
-    log("BS0", json.dumps(suite["BS0_launch"]))
+    # ---------------------------------------------- C++ drivers: BS0, BS6-BS8 per call
+    if not args.no_drivers:
+        suite["drivers"] = driver_suite()
+        suite["BS0_launch"] = suite["drivers"].pop("BS0_launch")
+        log("BS0", json.dumps(suite["BS0_launch"]))
+
+    # C1 output checksum on the GPU (BS8 on 8^3 p=7, seed 0), beside the CPU row's
+    g1 = ogs.Ogs.hex(8, 8, 8, 7, keep_ids=False)
+    q1 = DeviceArray(g1.N)
+    device.fill_uniform(q1, 0, 0)
+    g1.gather_scatter(q1)
+    c1_checksum = float(np.sum(q1.to_numpy().astype(np.longdouble)))
+    del g1, q1
 
     # ------------------------------------------- precision errors (config-1 and -2 sizes)
     suite["precision_errors"] = precision_errors()
@@ -561,6 +796,28 @@ def main():
     if not args.no_batched:
         suite["batched_64_meshes"] = batched_suite(K, W)
 
+    # ------------------------------------------------- HBM traffic from counters
+    traffic8, traffic_source = None, None
+    if not args.no_counters:
+        log("counter passes (rocprofv3 FETCH_SIZE, WRITE_SIZE) ...")
+        ctr, why = counter_suite(args)
+        if ctr is None:
+            log("counters:", why)
+            suite["counters"] = {"status": why}
+        else:
+            suite["counters"] = {"status": why, "calibration": ctr.pop("_calibration")}
+            for name, t in ctr.items():
+                d = suite.get(name)
+                if d is None:
+                    continue
+                d["traffic_bytes"] = int(t["traffic"])
+                d["traffic_over_algorithmic"] = round(t["traffic"] / d["bytes_per_call"], 3)
+                d["counter_GBs"] = round(t["traffic"] / (d["kernel_ms"] * 1e-3) / 1e9, 1)
+                d["counter_pct_spec"] = round(100 * d["counter_GBs"] / SPEC_HBM_GBS, 1)
+            if "BS8_gather_scatter" in ctr:
+                traffic8, traffic_source = ctr["BS8_gather_scatter"]["traffic"], why
+            log("counters", json.dumps({k: suite[k].get("counter_GBs") for k in ctr if k in suite}))
+
     # ---------------------------------------------------------------- report
     cpu = None
     if not args.no_cpu_baseline:
@@ -568,7 +825,9 @@ def main():
         cpu, cpu_all = cpu_baseline()
         suite["cpu_baseline_all_threads"] = cpu_all
         log("cpu baseline", json.dumps(cpu), json.dumps(cpu_all))
-    out = report(args, K, W, props, nx, p, Nl, Ng, n, bytes_bs8, host8, ev8, measured_peak, cpu, suite)
+        suite["cpu_rows"] = cpu_rows(c1_checksum)
+    out = report(args, K, W, props, nx, p, Nl, Ng, n, bytes_bs8, host8, ev8, measured_peak, cpu, suite, traffic8,
+                 traffic_source)
     print(json.dumps(out), flush=True)
 
 

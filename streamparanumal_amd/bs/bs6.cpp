@@ -52,6 +52,7 @@ int main(int argc, char **argv) {
     const double elapsedTime = ElapsedTime(start, end) / Ntests;
     const double kernelTime = dt.seconds() / Ntests;
     SPX_CHECK(spx_check_kernel_errors());
+    const double syncedTime = bs::timeSynced(platform, Ntests, [&] { mesh.ogs.Gather(o_gq, o_q, K, ogs::Add, ogs::Trans); });
 
     hlong NtotalGlobal = (hlong)mesh.Nelements * mesh.Np;
     mesh.comm.Allreduce(NtotalGlobal);
@@ -71,6 +72,7 @@ int main(int argc, char **argv) {
       printf("BS6device = [%d, " hlongFormat ", %5.4le, %5.4le, %6.2f]; %% Gather kernel-only [N, DOFs, elapsed, DOFs/s, BW (GB/s)], K=%d, block nnz %d\n",
              mesh.N, Ndofs, kernelTime, Ndofs / kernelTime, bytes / (1.0e9 * kernelTime), K,
              (int)mesh.ogs.localOperator()->blockNnz);
+      bs::printSynced("BS6", "Gather", mesh.N, Ndofs, syncedTime, bytes);
     }
   });
 }

@@ -52,6 +52,7 @@ int main(int argc, char **argv) {
     const double elapsedTime = ElapsedTime(start, end) / Ntests;
     const double kernelTime = dt.seconds() / Ntests;
     SPX_CHECK(spx_check_kernel_errors());
+    const double syncedTime = bs::timeSynced(platform, Ntests, [&] { mesh.ogs.Scatter(o_q, o_gq, K, ogs::NoTrans); });
 
     hlong NtotalGlobal = (hlong)mesh.Nelements * mesh.Np;
     mesh.comm.Allreduce(NtotalGlobal);
@@ -71,6 +72,7 @@ int main(int argc, char **argv) {
       printf("BS7device = [%d, " hlongFormat ", %5.4le, %5.4le, %6.2f]; %% Scatter kernel-only [N, DOFs, elapsed, DOFs/s, BW (GB/s)], K=%d, block nnz %d\n",
              mesh.N, Ndofs, kernelTime, Ndofs / kernelTime, bytes / (1.0e9 * kernelTime), K,
              (int)mesh.ogs.localOperator()->blockNnz);
+      bs::printSynced("BS7", "Scatter", mesh.N, Ndofs, syncedTime, bytes);
     }
   });
 }

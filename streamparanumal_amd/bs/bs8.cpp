@@ -67,6 +67,7 @@ int main(int argc, char **argv) {
     const double elapsedTime = ElapsedTime(start, end) / Ntests;
     const double kernelTime = dt.seconds() / Ntests;
     SPX_CHECK(spx_check_kernel_errors());
+    const double syncedTime = bs::timeSynced(platform, Ntests, step);
 
     hlong NtotalGlobal = (hlong)mesh.Nelements * mesh.Np;
     mesh.comm.Allreduce(NtotalGlobal);
@@ -86,6 +87,7 @@ int main(int argc, char **argv) {
       printf("BS8device = [%d, " hlongFormat ", %5.4le, %5.4le, %6.2f]; %% GatherScatter kernel-only [N, DOFs, elapsed, DOFs/s, BW (GB/s)], K=%d, block nnz %d, %s storage\n",
              mesh.N, Ndofs, kernelTime, Ndofs / kernelTime, bytes / (1.0e9 * kernelTime), K,
              (int)mesh.ogs.localOperator()->gsBlockNnz, settings.getSetting("STORAGE").c_str());
+      bs::printSynced("BS8", "GatherScatter", mesh.N, Ndofs, syncedTime, bytes);
     }
   });
 }

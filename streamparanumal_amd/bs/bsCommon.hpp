@@ -35,6 +35,31 @@ inline int run(const std::function<void()> &body) {
   }
 }
 
+// The reference's ogs_t waits on the device inside every Gather, Scatter and
+// GatherScatter call when MPI is not GPU-aware (libs/ogs/ogs.cpp:74,104,117,
+// 243, 435), so its BS6-BS8 loops time one synchronised call after another.
+// This ogs_t only enqueues (one rank has no halo to wait for), and the BSn
+// rows time back-to-back calls.  timeSynced repeats `step` with a host finish
+// after every call, which is what the reference's loop measures.
+inline double timeSynced(platform_t &platform, int Ntests, const std::function<void()> &step) {
+  step();
+  platform.finish();
+  timePoint_t start = GlobalPlatformTime(platform);
+  for (int n = 0; n < Ntests; ++n) {
+    step();
+    platform.finish();
+  }
+  timePoint_t end = GlobalPlatformTime(platform);
+  return ElapsedTime(start, end) / Ntests;
+}
+
+// The BSnsync row: [N, DOFs, elapsed per synchronised call, DOFs/s, BW (GB/s)].
+inline void printSynced(const char *name, const char *what, int N, hlong Ndofs, double t, size_t bytes) {
+  printf("%ssync = [%d, " hlongFormat ", %5.4le, %5.4le, %6.2f]; %% %s, host finish after every call as in the "
+         "reference's ogs_t [N, DOFs, elapsed, DOFs/s, BW (GB/s)] (the %s row times back-to-back calls)\n",
+         name, N, Ndofs, t, Ndofs / t, bytes / (1.0e9 * t), what, name);
+}
+
 inline void addVerbose(settings_t &s) {
   s.newSetting("-v", "--verbose", "VERBOSE", "FALSE", "Enable verbose output", {"TRUE", "FALSE"});
 }
