@@ -40,21 +40,21 @@ BIN = os.path.join(ROOT, "streamparanumal_amd", "bin")
 KERNEL_PATTERNS = {
     "BS1_copy": r"k_copy2<4, true>",
     "BS2_axpy": r"k_axpy2<",
-    "BS3_norm_reference": r"k_red_ref1<0,",
-    "BS4_dot_reference": r"k_red_ref1<1,",
-    "BS5_cg_update_reference": r"k_red_ref1<2,",
-    "BS5_cg_full_reference": r"k_red_ref1<3,",
-    "BS3_norm_fast": r"k_red_fast1<0,",
-    "BS4_dot_fast": r"k_red_fast1<1,",
-    "BS5_cg_update_fast": r"k_red_fast1<2,",
-    "BS5_cg_full_fast": r"k_red_fast1<3,",
+    "BS3_norm_reference": r"k_red_ref1<0, \d+, \d+, \w+, false>",
+    "BS4_dot_reference": r"k_red_ref1<1, \d+, \d+, \w+, false>",
+    "BS5_cg_update_reference": r"k_red_ref1<2, \d+, \d+, \w+, false>",
+    "BS5_cg_full_reference": r"k_red_ref1<3, \d+, \d+, \w+, false>",
+    "BS3_norm_fast": r"k_red_ref1<0, \d+, \d+, \w+, true>",
+    "BS4_dot_fast": r"k_red_ref1<1, \d+, \d+, \w+, true>",
+    "BS5_cg_update_fast": r"k_red_ref1<2, \d+, \d+, \w+, true>",
+    "BS5_cg_full_fast": r"k_red_ref1<3, \d+, \d+, \w+, true>",
     "BS5_cg_full_f32": r"k_cg_mixed1<float",
-    "BS5_cg_full_bf16": r"k_cg_mixed1<unsigned short",
+    "BS5_cg_full_bf16": r"k_cg_mixed1<[^>]*bf16_t",
     "BS6_gather": r"k_gather_srb<double",
     "BS7_scatter": r"k_scatter_srb<double",
     "BS8_gather_scatter": r"k_gs_srb<double",
     "BS8_gather_scatter_f32": r"k_gs_srb<float",
-    "BS8_gather_scatter_bf16": r"k_gs_srb<unsigned short",
+    "BS8_gather_scatter_bf16": r"k_gs_srb<[^,]*bf16_t",
 }
 
 
@@ -121,9 +121,11 @@ def read_traffic():
     return float(m.group(1)), f"profiles/{os.path.basename(PROFILE_COUNTERS)} (same kernel sources)"
 
 
-def traffic_probe(nx, p, log2n, reps=3):
+def traffic_probe(nx, p, log2n, reps=3, batched=False):
     """The launches the counter passes measure (run as the profiled child process): the plain
-    calibration copy of 2^27 doubles, then `reps` calls of every suite kernel on the bench's inputs."""
+    calibration copy of 2^27 doubles, then `reps` calls of every suite kernel on the bench's inputs
+    (batched: instead, `reps` steps of the 64-mesh batch in FP32 and in FP64 with the default
+    launch shapes)."""
     import numpy as np
 
     import streamparanumal_amd as spx
@@ -131,6 +133,25 @@ def traffic_probe(nx, p, log2n, reps=3):
     from streamparanumal_amd.device import DeviceArray
 
     ogs.platform_open(0)
+    if batched:
+        from streamparanumal_amd.batch import Batch, instances
+        c0, c1 = DeviceArray(1 << 27), DeviceArray(1 << 27)
+        device.fill_uniform(c0, 1, 0)
+        device.set_tuning("bs1", unroll=4, nontemporal=0)
+        for _ in range(reps):
+            streaming.copy(c0, c1)
+        device.set_tuning("bs1")
+        del c0, c1
+        b = Batch(instances(64, 4), 3, np.float32)
+        for bb in (b, b.sibling(np.float64)):
+            bb.allocate(seed=5)
+            bb.make_plan()
+            for _ in range(reps):
+                bb.step(0.37)
+            device.finish()
+            bb.vectors = None
+        device.check_kernel_errors()
+        return
     n = 1 << log2n
     a, b, pv, xv = (DeviceArray(n) for _ in range(4))
     for i, v in enumerate((a, b, pv, xv)):
@@ -218,7 +239,8 @@ def driver_suite():
     """Rows of the C++ drivers: BS0 as 10^4 empty launches, plain and replayed from a graph, timed on
     the host after a finish as bs0Run.cpp:70-77 does; BS6-BS8 at 8^3 p=7 (seed 0) and 64^3 p=7
     (seed 1) with three timings per call: back-to-back calls (the BSn row), kernel-only events, and
-    a host finish after every call, which is what the reference's ogs_t does inside each call."""
+    a host finish after every call, which is what the reference's ogs_t does inside each call; each
+    after the same untimed settle as the bench's own timings."""
     out = {}
     plain = run_driver("BS0", "-n", "10000")["BS0"]
     graph = run_driver("BS0", "-n", "10000", "--graph", "TRUE")["BS0"]
@@ -226,7 +248,8 @@ def driver_suite():
                          "graph_us_per_launch": round(graph[1] * 1e6, 3), "timer": "C++ driver bin/BS0"}
     for tag, nx, seed in (("8^3 p=7", 8, 0), ("64^3 p=7", 64, 1)):
         for n, what in (("BS6", "gather"), ("BS7", "scatter"), ("BS8", "gather_scatter")):
-            rows = run_driver(n, "-nx", str(nx), "-ny", str(nx), "-nz", str(nx), "-p", "7", "--seed", str(seed))
+            rows = run_driver(n, "-nx", str(nx), "-ny", str(nx), "-nz", str(nx), "-p", "7", "--seed", str(seed),
+                              "--settle", str(SETTLE_SECONDS))
             b2b, dev, syn = rows[n], rows[n + "device"], rows[n + "sync"]
             dofs = b2b[1]
             out[f"{n}_{what}_{nx}cube_p7"] = {
@@ -600,9 +623,12 @@ def main():
     ap.add_argument("--no-drivers", action="store_true", help="skip the C++ driver rows (BS0, BS6-BS8 per call)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="internal: the launches the counter passes profile (run by bench.py itself)")
+    ap.add_argument("--probe-batched", action="store_true",
+                    help="with --traffic-probe: steps of the 64-mesh batch instead of the suite kernels (the "
+                         "command of the committed counter summary of the batched kernels)")
     args = ap.parse_args()
     if args.traffic_probe:
-        traffic_probe(args.nx, args.degree, args.log2n)
+        traffic_probe(args.nx, args.degree, args.log2n, batched=args.probe_batched)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "

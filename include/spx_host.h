@@ -156,6 +156,16 @@ int spxh_csr_destroy(spxh_csr_t csr);
 /* row blocks of at most cap entries (greedy); blockStarts holds nrows+2 */
 int spxh_block_rows(int32_t nrows, const int32_t *rowStarts, int32_t cap, int32_t *nblocks,
                     int32_t *blockStarts);
+/* Line-aligned gather+scatter tiles of an nx x ny x nz hex box at degree p
+   (no reference counterpart): row_grid[r] is the lattice index (x fastest)
+   of T-CSR row r's point.  order receives the rows tile by tile and cuts[i]
+   is 1 where a tile starts (spxh_ogs_set_gs_tiles's form).  A tile holds the
+   rows of one z plane, full x lines, and y points that never split a group
+   of rows_per_line node rows of an element (the memory line of the vector),
+   merged up to cap members; larger groups are cut along x at element
+   boundaries. */
+int spxh_hex_gs_tiles(int nx, int ny, int nz, int p, int rows_per_line, int cap, int32_t nrows,
+                      const int64_t *row_grid, int32_t *order, uint8_t *cuts);
 
 /* Sorted-run tiles (spx_srb_t) of a CSR and its row blocks, on the host (no
    device needed): sizes[0..5] = nblocks, nnz, nchunks (= nblocks *

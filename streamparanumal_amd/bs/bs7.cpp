@@ -11,6 +11,7 @@ int main(int argc, char **argv) {
     platformAddSettings(settings);
     meshAddSettings(settings);
     bs::addVerbose(settings);
+    bs::addSettle(settings);
     bs::addSeed(settings);
     settings.newSetting("--fields", "--fields", "FIELDS", "1", "Fields per node (K)");
     settings.newSetting("--block-nnz", "--block-nnz", "BLOCK NNZ", "0", "Max column entries per row block (0: library default)");
@@ -41,6 +42,7 @@ int main(int argc, char **argv) {
     deviceMemory<dfloat> o_gq = platform.malloc<dfloat>((size_t)Ngather * K);
     bs::fillUniform(o_gq, seed, 0);
     for (int n = 0; n < 5; ++n) { mesh.ogs.Scatter(o_q, o_gq, K, ogs::NoTrans); }
+    bs::settle(settings, platform, [&] { mesh.ogs.Scatter(o_q, o_gq, K, ogs::NoTrans); });
 
     const int Ntests = 20;
     deviceTimer_t dt;

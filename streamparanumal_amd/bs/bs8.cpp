@@ -13,6 +13,7 @@ int main(int argc, char **argv) {
     platformAddSettings(settings);
     meshAddSettings(settings);
     bs::addVerbose(settings);
+    bs::addSettle(settings);
     bs::addSeed(settings);
     settings.newSetting("--fields", "--fields", "FIELDS", "1", "Fields per node (K)");
     settings.newSetting("--block-nnz", "--block-nnz", "BLOCK NNZ", "0", "Max column entries per row block (0: library default)");
@@ -56,6 +57,7 @@ int main(int argc, char **argv) {
         mesh.ogs.localOperator()->GatherScatterMixedRaw(storage, K, s_q.ptr());
     };
     for (int n = 0; n < 5; ++n) step();
+    bs::settle(settings, platform, step);
 
     const int Ntests = 50;
     deviceTimer_t dt;

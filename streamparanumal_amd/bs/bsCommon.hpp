@@ -53,6 +53,22 @@ inline double timeSynced(platform_t &platform, int Ntests, const std::function<v
   return ElapsedTime(start, end) / Ntests;
 }
 
+// --settle S (seconds, default 0 = the reference's 5 dry runs only): repeat
+// the step untimed for S seconds first, so the timed loops see the GPU at its
+// sustained clocks rather than ramping up after the host-side setup.
+inline void addSettle(settings_t &s) {
+  s.newSetting("--settle", "--settle", "SETTLE", "0", "Seconds of untimed calls before timing");
+}
+inline void settle(settings_t &s, platform_t &platform, const std::function<void()> &step) {
+  double seconds = 0.0;
+  s.getSetting("SETTLE", seconds);
+  if (seconds <= 0.0) return;
+  const timePoint_t t0 = PlatformTime(platform);
+  do {
+    for (int n = 0; n < 8; ++n) step();
+  } while (ElapsedTime(t0, PlatformTime(platform)) < seconds);
+}
+
 // The BSnsync row: [N, DOFs, elapsed per synchronised call, DOFs/s, BW (GB/s)].
 inline void printSynced(const char *name, const char *what, int N, hlong Ndofs, double t, size_t bytes) {
   printf("%ssync = [%d, " hlongFormat ", %5.4le, %5.4le, %6.2f]; %% %s, host finish after every call as in the "

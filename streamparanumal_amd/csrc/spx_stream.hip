@@ -9,9 +9,9 @@
 //               LDS tree down to s[0]+s[1], and a second one-block pass.  The
 //               serial chain per thread is kept; only its loads are issued
 //               ahead (U independent loads, then U dependent adds).
-//   FAST      - a fixed grid over the whole machine, compensated (Neumaier)
-//               per-thread sums, a fixed tree, and a one-block second pass.
-//               Deterministic run to run; within 1e-14 of the reference order.
+//   FAST      - the same launch shape and element order with compensated
+//               (Neumaier) lane sums and a compensated second pass: as fast as
+//               the reference order, deterministic, and within 1e-14 of it.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -111,7 +111,19 @@ struct RedArgs {
   double *r;         // CG, CGFULL: r (updated)
   double *x;         // CGFULL: x (updated)
   double alpha;
+  // CGFULL: the x update of index i runs beside the r update of index
+  // i + n - xshift (mod n): x and p then stream at another address phase
+  // than r and Ap.  Each x entry is still updated once, from its own p, so
+  // results do not change.
+  int64_t xshift = 0;
+  int64_t n = 0;
 };
+
+// CGFULL's x index for the r index i (see RedArgs::xshift)
+__device__ __forceinline__ int64_t x_index(const RedArgs &A, int64_t i) {
+  const int64_t j = i + A.xshift;
+  return j >= A.n ? j - A.n : j;
+}
 
 // One term of the reduction at element i, applying the update when the
 // kernel has one.  Loads of term i are independent of the running sum.
@@ -127,7 +139,8 @@ __device__ __forceinline__ double red_term(const RedArgs &A, int64_t i) {
     A.r[i] = rr;
     return rr * rr;
   } else {
-    A.x[i] = A.x[i] + A.alpha * A.a[i];
+    const int64_t ix = x_index(A, i);
+    A.x[ix] = A.x[ix] + A.alpha * A.a[ix];
     const double rr = A.r[i] - A.alpha * A.b[i];
     A.r[i] = rr;
     return rr * rr;
@@ -159,16 +172,45 @@ template <bool NT> __device__ __forceinline__ void stv(double *p, double v) {
   else *p = v;
 }
 
-template <int KIND, int U, bool NT>
-__device__ __forceinline__ void ref_group(const RedArgs &A, int64_t id, int64_t stride, double &acc) {
+// FAST mode's compensated (Neumaier) running sum: sum + c carries the
+// rounding error of every add; the lane's result is sum + c.
+__device__ __forceinline__ void neumaier(double &sum, double &c, double v) {
+  const double t = sum + v;
+  if (fabs(sum) >= fabs(v)) c += (sum - t) + v;
+  else c += (v - t) + sum;
+  sum = t;
+}
+
+// The running sum of one lane: plain adds in index order (the reference's
+// serial chain) or compensated.
+template <bool COMP> struct LaneSum;
+template <> struct LaneSum<false> {
+  double s = 0.0;
+  __device__ __forceinline__ void add(double v) { s += v; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+template <> struct LaneSum<true> {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double v) { neumaier(s, c, v); }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
+template <int KIND, int U, bool NT, typename ACC>
+__device__ __forceinline__ void ref_group(const RedArgs &A, int64_t id, int64_t stride, ACC &acc) {
   double va[U], vb[U], vr[U], vx[U];
+  int64_t ix[KIND == RED_CGFULL ? U : 1];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int64_t i = id + u * stride;
-    va[u] = ldv<NT>(A.a + i);
+    if constexpr (KIND == RED_CGFULL) {
+      ix[u] = x_index(A, i);
+      va[u] = ldv<NT>(A.a + ix[u]);
+    } else {
+      va[u] = ldv<NT>(A.a + i);
+    }
     if constexpr (KIND == RED_DOT || KIND == RED_CGFULL) vb[u] = ldv<NT>(A.b + i);
     if constexpr (KIND == RED_CG || KIND == RED_CGFULL) vr[u] = ldv<NT>(A.r + i);
-    if constexpr (KIND == RED_CGFULL) vx[u] = ldv<NT>(A.x + i);
+    if constexpr (KIND == RED_CGFULL) vx[u] = ldv<NT>(A.x + ix[u]);
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -183,27 +225,30 @@ __device__ __forceinline__ void ref_group(const RedArgs &A, int64_t id, int64_t 
       stv<NT>(A.r + i, rr);
       term = rr * rr;
     } else {
-      stv<NT>(A.x + i, vx[u] + A.alpha * va[u]);
+      stv<NT>(A.x + ix[u], vx[u] + A.alpha * va[u]);
       const double rr = vr[u] - A.alpha * vb[u];
       stv<NT>(A.r + i, rr);
       term = rr * rr;
     }
-    acc += term;
+    acc.add(term);
   }
 }
 
-template <int KIND, int BS, int U, bool NT>
+// COMP = false: the reference order exactly.  COMP = true (FAST mode): the
+// same launch shape and element order with compensated lane sums, so it runs
+// at the reference order's bandwidth and is more accurate.
+template <int KIND, int BS, int U, bool NT, bool COMP = false>
 __global__ __launch_bounds__(BS) void k_red_ref1(int64_t n, int64_t nblock, RedArgs A,
                                                  double *__restrict__ partials) {
   __shared__ double s[BS];
   const int t = threadIdx.x;
   const int64_t stride = (int64_t)BS * nblock;
   int64_t id = t + (int64_t)blockIdx.x * BS;
-  double acc = 0.0;
+  LaneSum<COMP> acc;
   for (; id + (int64_t)(U - 1) * stride < n; id += (int64_t)U * stride)
     ref_group<KIND, U, NT>(A, id, stride, acc);
-  for (; id < n; id += stride) acc += red_term<KIND>(A, id);
-  s[t] = acc;
+  for (; id < n; id += stride) acc.add(red_term<KIND>(A, id));
+  s[t] = acc.value();
   __syncthreads();
   const double blk = ref_tree<BS>(s);
   if (t == 0) partials[blockIdx.x] = blk;
@@ -223,54 +268,6 @@ __global__ __launch_bounds__(BS) void k_red_ref2(int64_t nblock, const double *_
   if (t == 0) out[0] = v;
 }
 
-// FAST stage 1: grid of G blocks x 256, 16-byte loads, Neumaier sums.
-__device__ __forceinline__ void neumaier(double &sum, double &c, double v) {
-  const double t = sum + v;
-  if (fabs(sum) >= fabs(v)) c += (sum - t) + v;
-  else c += (v - t) + sum;
-  sum = t;
-}
-
-template <int KIND>
-__device__ __forceinline__ void fast_pair(const RedArgs &A, int64_t i2, double &s, double &c) {
-  const int64_t i = 2 * i2;
-  if constexpr (KIND == RED_NORM) {
-    const dbl2 v = *reinterpret_cast<const dbl2 *>(A.a + i);
-    neumaier(s, c, v.x * v.x);
-    neumaier(s, c, v.y * v.y);
-  } else if constexpr (KIND == RED_DOT) {
-    const dbl2 v = *reinterpret_cast<const dbl2 *>(A.a + i);
-    const dbl2 w = *reinterpret_cast<const dbl2 *>(A.b + i);
-    neumaier(s, c, v.x * w.x);
-    neumaier(s, c, v.y * w.y);
-  } else if constexpr (KIND == RED_CG) {
-    const dbl2 ap = *reinterpret_cast<const dbl2 *>(A.a + i);
-    dbl2 r = *reinterpret_cast<const dbl2 *>(A.r + i);
-    r.x = r.x + A.alpha * ap.x;
-    r.y = r.y + A.alpha * ap.y;
-    *reinterpret_cast<dbl2 *>(A.r + i) = r;
-    neumaier(s, c, r.x * r.x);
-    neumaier(s, c, r.y * r.y);
-  } else {
-    const dbl2 p = *reinterpret_cast<const dbl2 *>(A.a + i);
-    const dbl2 ap = *reinterpret_cast<const dbl2 *>(A.b + i);
-    dbl2 x = *reinterpret_cast<const dbl2 *>(A.x + i);
-    dbl2 r = *reinterpret_cast<const dbl2 *>(A.r + i);
-    x.x = x.x + A.alpha * p.x;
-    x.y = x.y + A.alpha * p.y;
-    r.x = r.x - A.alpha * ap.x;
-    r.y = r.y - A.alpha * ap.y;
-    *reinterpret_cast<dbl2 *>(A.x + i) = x;
-    *reinterpret_cast<dbl2 *>(A.r + i) = r;
-    neumaier(s, c, r.x * r.x);
-    neumaier(s, c, r.y * r.y);
-  }
-}
-
-template <int KIND>
-__device__ __forceinline__ void fast_one(const RedArgs &A, int64_t i, double &s, double &c) {
-  neumaier(s, c, red_term<KIND>(A, i));
-}
 
 // Block sum of (s + c) over 256 threads with a fixed shape: wave-level
 // shuffles then LDS across the 4 waves.
@@ -286,69 +283,7 @@ __device__ __forceinline__ double block_sum256(double v) {
   return out;
 }
 
-// U pairs per lane: every load of the group first (streaming when NT), then the
-// updates, stores and compensated adds in (u, .x, .y) order.
-template <int KIND, int U, bool NT>
-__device__ __forceinline__ void fast_group(const RedArgs &A, int64_t i2, int64_t stride, double &s, double &c) {
-  dbl2 va[U], vb[U], vr[U], vx[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int64_t i = 2 * (i2 + u * stride);
-    va[u] = ld2<NT>(reinterpret_cast<const dbl2 *>(A.a + i));
-    if constexpr (KIND == RED_DOT || KIND == RED_CGFULL) vb[u] = ld2<NT>(reinterpret_cast<const dbl2 *>(A.b + i));
-    if constexpr (KIND == RED_CG || KIND == RED_CGFULL) vr[u] = ld2<NT>(reinterpret_cast<const dbl2 *>(A.r + i));
-    if constexpr (KIND == RED_CGFULL) vx[u] = ld2<NT>(reinterpret_cast<const dbl2 *>(A.x + i));
-  }
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int64_t i = 2 * (i2 + u * stride);
-    if constexpr (KIND == RED_NORM) {
-      neumaier(s, c, va[u].x * va[u].x);
-      neumaier(s, c, va[u].y * va[u].y);
-    } else if constexpr (KIND == RED_DOT) {
-      neumaier(s, c, va[u].x * vb[u].x);
-      neumaier(s, c, va[u].y * vb[u].y);
-    } else if constexpr (KIND == RED_CG) {
-      dbl2 r = vr[u];
-      r.x = r.x + A.alpha * va[u].x;
-      r.y = r.y + A.alpha * va[u].y;
-      st2<NT>(reinterpret_cast<dbl2 *>(A.r + i), r);
-      neumaier(s, c, r.x * r.x);
-      neumaier(s, c, r.y * r.y);
-    } else {
-      dbl2 x = vx[u], r = vr[u];
-      x.x = x.x + A.alpha * va[u].x;
-      x.y = x.y + A.alpha * va[u].y;
-      r.x = r.x - A.alpha * vb[u].x;
-      r.y = r.y - A.alpha * vb[u].y;
-      st2<NT>(reinterpret_cast<dbl2 *>(A.x + i), x);
-      st2<NT>(reinterpret_cast<dbl2 *>(A.r + i), r);
-      neumaier(s, c, r.x * r.x);
-      neumaier(s, c, r.y * r.y);
-    }
-  }
-}
-
-template <int KIND, int U, bool NT>
-__global__ __launch_bounds__(256) void k_red_fast1(int64_t n, RedArgs A, double *__restrict__ partials,
-                                                   int vec) {
-  double s = 0.0, c = 0.0;
-  const int64_t nthreads = (int64_t)gridDim.x * 256;
-  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (vec) {
-    const int64_t n2 = n / 2;
-    int64_t i2 = tid;
-    for (; i2 + (int64_t)(U - 1) * nthreads < n2; i2 += (int64_t)U * nthreads)
-      fast_group<KIND, U, NT>(A, i2, nthreads, s, c);
-    for (; i2 < n2; i2 += nthreads) fast_pair<KIND>(A, i2, s, c);
-    if (tid == 0 && (n & 1)) fast_one<KIND>(A, n - 1, s, c);
-  } else {
-    for (int64_t i = tid; i < n; i += nthreads) fast_one<KIND>(A, i, s, c);
-  }
-  const double b = block_sum256(s + c);
-  if (threadIdx.x == 0) partials[blockIdx.x] = b;
-}
-
+// FAST stage 2: compensated sum of the partials, then the fixed block tree.
 __global__ __launch_bounds__(256) void k_red_fast2(int64_t g, const double *__restrict__ partials,
                                                    double *__restrict__ out) {
   double s = 0.0, c = 0.0;
@@ -392,28 +327,33 @@ static void launch_axpy2(unsigned grid, int64_t n2, double alpha, const double *
 // reports kFastGridMax for FAST mode, so a partials buffer sized by it stays
 // large enough whatever tuning is set later.
 constexpr int kFastGridMax = 4096;
+// Measured on MI355X at N = 2^27 (TB/s, norm / dot / update / full form):
+// the reference loop shape with compensated lane sums runs like the
+// reference order itself; an earlier FAST kernel (16-byte loads, 8 per lane,
+// on 4096 blocks) gave 5.6 / 5.5 / 4.3 / 4.9 and on 256 blocks 7.2 / 7.1 /
+// 4.4 / 5.6, against the reference order's 7.0 / 7.0 / 6.0 / 5.8.
 static int fast_grid() {
   spx_tuning_t t = tuning_for("reduce_fast", stream_defaults());
-  int g = t.grid_blocks > 0 ? t.grid_blocks : 4096;  // measured: 5.6 TB/s norm at 4096 blocks, 5.4 at 2048
+  int g = t.grid_blocks > 0 ? t.grid_blocks : 256;
   return g < 1 ? 1 : (g > kFastGridMax ? kFastGridMax : g);
 }
 
-template <int KIND, int BS, bool NT>
+template <int KIND, int BS, bool NT, bool COMP>
 static void launch_ref1_nt(int64_t n, int64_t nb, const RedArgs &A, double *tmp, int unroll) {
   if (unroll >= 32)
-    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 32, NT>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n, nb, A,
-                       tmp);
+    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 32, NT, COMP>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n,
+                       nb, A, tmp);
   else if (unroll >= 16)
-    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 16, NT>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n, nb, A,
-                       tmp);
+    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 16, NT, COMP>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n,
+                       nb, A, tmp);
   else
-    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 8, NT>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n, nb, A,
-                       tmp);
+    hipLaunchKernelGGL((k_red_ref1<KIND, BS, 8, NT, COMP>), dim3((unsigned)nb), dim3(BS), 0, current_stream(), n,
+                       nb, A, tmp);
 }
-template <int KIND, int BS>
+template <int KIND, int BS, bool COMP = false>
 static void launch_ref1(int64_t n, int64_t nb, const RedArgs &A, double *tmp, int unroll, bool nt) {
-  if (nt) launch_ref1_nt<KIND, BS, true>(n, nb, A, tmp, unroll);
-  else launch_ref1_nt<KIND, BS, false>(n, nb, A, tmp, unroll);
+  if (nt) launch_ref1_nt<KIND, BS, true, COMP>(n, nb, A, tmp, unroll);
+  else launch_ref1_nt<KIND, BS, false, COMP>(n, nb, A, tmp, unroll);
 }
 
 // Measured on MI355X at N = 2^27 (TB/s, unroll 16): streaming loads/stores
@@ -459,20 +399,37 @@ static int run_reduction(const char *name, int64_t n, const RedArgs &A, double *
     return SPX_OK;
   }
   if (mode != SPX_REDUCE_FAST) return fail(SPX_ERR_ARG, std::string(name) + ": unknown reduction mode");
-  const int g = fast_grid();
-  bool vec = aligned(A.a, 16) && (!A.b || aligned(A.b, 16)) && (!A.r || aligned(A.r, 16)) &&
-             (!A.x || aligned(A.x, 16));
-  const spx_tuning_t ft = tuning_for("reduce_fast", stream_defaults());
-  if (ft.nontemporal > 0)
-    hipLaunchKernelGGL((k_red_fast1<KIND, 8, true>), dim3(g), dim3(256), 0, current_stream(), n, A, tmp,
-                       vec ? 1 : 0);
-  else
-    hipLaunchKernelGGL((k_red_fast1<KIND, 8, false>), dim3(g), dim3(256), 0, current_stream(), n, A, tmp,
-                       vec ? 1 : 0);
-  SPX_LAUNCHED(name);
-  hipLaunchKernelGGL(k_red_fast2, dim3(1), dim3(256), 0, current_stream(), (int64_t)g, tmp, out);
+  // FAST: min(ceil(n/256), grid) blocks of 256 lanes in the reference's
+  // element order, compensated lane sums, then a compensated fold of the
+  // partials; deterministic for a given grid.
+  int64_t nb = (n + 255) / 256;
+  if (nb > fast_grid()) nb = fast_grid();
+  spx_tuning_t fd = reference_tuning(KIND);  // loads per lane and streaming accesses as the reference order's
+  fd.grid_blocks = 0;
+  const spx_tuning_t ft = tuning_for("reduce_fast", fd);
+  const int U = ft.unroll > 0 ? ft.unroll : 16;
+  if (nb > 0) {
+    launch_ref1<KIND, 256, true>(n, nb, A, tmp, U, ft.nontemporal > 0);
+    SPX_LAUNCHED(name);
+  }
+  hipLaunchKernelGGL(k_red_fast2, dim3(1), dim3(256), 0, current_stream(), nb, tmp, out);
   SPX_LAUNCHED(name);
   return SPX_OK;
+}
+
+// The full form's x/p address phase shift (RedArgs::xshift), in entries.
+// Default: half the vector, rounded down to 64 entries.  Measured on MI355X,
+// N = 2^27, the four vectors carved 1 GiB + d apart (TB/s, no shift / half):
+// d = 0 5.23 / 5.76, 512 B 5.53 / 5.74, 4 KiB 5.80 / 5.75, 64 KiB 5.33 /
+// 5.70, 2 MiB 5.71 / 5.82, separate hipMalloc blocks 5.51 / 5.78 (copy 6.24).
+// Tuning "bs5_full": variant v > 0 shifts by v*64 entries (mod n).
+static int64_t full_xshift(int64_t n) {
+  spx_tuning_t d{};
+  d.variant = 0;
+  const spx_tuning_t t = tuning_for("bs5_full", d);
+  if (n <= 0) return 0;
+  const int64_t sh = t.variant > 0 ? (int64_t)t.variant * 64 : (n / 2) / 64 * 64;
+  return sh % n;
 }
 
 // One stage at a time, with the caller's block count, exactly as the OKL
@@ -636,7 +593,7 @@ int spx_bs5_update(int64_t n, const double *Ap, double alpha, double *r, double 
 int spx_bs5_full(int64_t n, double alpha, const double *p, const double *Ap, double *x, double *r,
                  double *tmp, double *rdotr, int mode, int bs) {
   if (n > 0 && (!p || !Ap || !x || !r)) return fail(SPX_ERR_ARG, "spx_bs5_full: null input");
-  RedArgs A{p, Ap, r, x, alpha};
+  RedArgs A{p, Ap, r, x, alpha, full_xshift(n), n};
   return run_reduction<RED_CGFULL>("spx_bs5_full", n, A, tmp, rdotr, mode, bs);
 }
 

@@ -233,6 +233,16 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
 // Block shape measured best on MI355X for a hex box of degree p (64^3 p=7 and
 // ~1e8-node boxes at p = 1..15).
 void HexGsBlockShape(const int p, int &bx, int &by, int &bz);
+// Gather+scatter tiles of an nx x ny x nz hex box at degree p whose members
+// fill whole memory lines of the vector: a line holds rowsPerLine node rows
+// (fixed j, k) of one element, so a tile takes every row of a full x line of
+// grid points for a set of y points that never splits such a group (the last
+// rows of element ey and the first of ey+1 share a point and go together),
+// in one z plane; groups are merged up to cap members per tile, and a group
+// above cap is cut along x at element boundaries.  order: the rows, tile by
+// tile; cuts: 1 where a tile starts (for SetGsRowOrder).
+void HexLineTiles(const dlong Nrows, const memory<hlong> &rowGrid, const hlong nx, const hlong ny, const hlong nz,
+                  const int p, const int rowsPerLine, const int cap, memory<dlong> &order, memory<uint8_t> &cuts);
 
 // Parallel exclusive prefix sum of v[0..n) in place; returns the total.  The
 // slices follow the team the runtime actually gives, which may be smaller than

@@ -540,6 +540,23 @@ int spxh_block_rows(int32_t nrows, const int32_t *rowStarts, int32_t cap, int32_
   });
 }
 
+int spxh_hex_gs_tiles(int nx, int ny, int nz, int p, int rows_per_line, int cap, int32_t nrows,
+                      const int64_t *row_grid, int32_t *order, uint8_t *cuts) {
+  if (nrows < 0 || (nrows > 0 && (!row_grid || !order || !cuts)))
+    return err(SPX_ERR_ARG, "spxh_hex_gs_tiles: bad arguments");
+  return guarded([&] {
+    memory<hlong> grid(nrows > 0 ? nrows : 1);
+    for (int32_t r = 0; r < nrows; ++r) grid[r] = row_grid[r];
+    memory<dlong> o;
+    memory<uint8_t> c;
+    ogs::HexLineTiles(nrows, grid, nx, ny, nz, p, rows_per_line, cap, o, c);
+    if (nrows > 0) {
+      std::copy(o.ptr(), o.ptr() + nrows, order);
+      std::copy(c.ptr(), c.ptr() + nrows, cuts);
+    }
+  });
+}
+
 int spxh_mesh_hex_ids(int nx, int ny, int nz, int p, int64_t *ids) {
   if (!ids) return err(SPX_ERR_ARG, "spxh_mesh_hex_ids: null ids");
   if (nx < 1 || ny < 1 || nz < 1 || p < 1 || p > 15) return err(SPX_ERR_ARG, "spxh_mesh_hex_ids: bad size");

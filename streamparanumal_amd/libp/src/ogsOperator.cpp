@@ -137,6 +137,9 @@ void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order, const bool validat
   BuildGsSrb(gsSrb.xcdRemap, gsSrb.tileLanes);
 }
 
+// Stable counting sort of rows by key[r] in [0, nkeys).
+static memory<dlong> CountingOrder(const dlong Nrows, const std::vector<int32_t> &key, const hlong nkeys);
+
 memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid, const hlong Mx, const hlong My,
                                 const hlong Mz, const int bx, const int by, const int bz) {
   LIBP_ABORT("lattice blocks need positive sizes", Mx < 1 || My < 1 || Mz < 1 || bx < 1 || by < 1 || bz < 1);
@@ -144,7 +147,6 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
   const hlong nkeys = nbx * nby * nbz;
   LIBP_ABORT("too many lattice blocks (" << nkeys << ")", nkeys > (hlong)INT32_MAX);
   if (Nrows <= 0) return memory<dlong>();  // no rows: the CSR's (empty) order
-  memory<dlong> order(Nrows);
   std::vector<int32_t> key(Nrows);
 #pragma omp parallel for schedule(static)
   for (dlong r = 0; r < Nrows; ++r) {
@@ -152,6 +154,79 @@ memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid,
     const hlong i = g % Mx, j = (g / Mx) % My, k = g / (Mx * My);
     key[r] = (int32_t)(((k / bz) * nby + j / by) * nbx + i / bx);
   }
+  return CountingOrder(Nrows, key, nkeys);
+}
+
+void HexLineTiles(const dlong Nrows, const memory<hlong> &rowGrid, const hlong nx, const hlong ny, const hlong nz,
+                  const int p, const int rowsPerLine, const int cap, memory<dlong> &order, memory<uint8_t> &cuts) {
+  LIBP_ABORT("line tiles need a box, a degree and a capacity",
+             nx < 1 || ny < 1 || nz < 1 || p < 1 || rowsPerLine < 1 || cap < 1);
+  order = memory<dlong>();
+  cuts = memory<uint8_t>();
+  if (Nrows <= 0) return;
+  const hlong Mx = nx * p + 1, My = ny * p + 1, Mz = nz * p + 1;
+  auto mult = [p](hlong a, hlong n) -> hlong { return (a % p == 0 && a > 0 && a < n * p) ? 2 : 1; };
+  // y groups: an element's node rows j = 0..p in runs of rowsPerLine; the
+  // last run of element ey and the first of ey+1 share the point p*(ey+1)
+  // and merge, unless that run of ey is the shared point alone (p a multiple
+  // of rowsPerLine), which then starts a group
+  std::vector<int32_t> yg(My);
+  int32_t ng = 0;
+  for (hlong y = 0; y < My; ++y) {
+    const hlong j = y % p;
+    if (y == 0 || (j != 0 && j % rowsPerLine == 0) || (j == 0 && p % rowsPerLine == 0)) ++ng;
+    yg[y] = ng - 1;
+  }
+  std::vector<hlong> ysl(ng, 0);  // y slots (members along y) per group
+  for (hlong y = 0; y < My; ++y) ysl[yg[y]] += mult(y, ny);
+  const hlong xsl = nx * (p + 1);  // members along a full x line
+  // per z plane: consecutive groups merged while a tile holds at most cap
+  // members; a group larger than cap is cut along x at element boundaries
+  std::vector<int32_t> tileOf((size_t)Mz * ng), chunksOf((size_t)Mz * ng);
+  std::vector<int64_t> zBase(Mz + 1, 0);
+  for (hlong z = 0; z < Mz; ++z) {
+    int64_t tiles = 0, acc = 0;
+    for (int32_t g = 0; g < ng; ++g) {
+      const int64_t P = xsl * ysl[g] * mult(z, nz);
+      const size_t zg = (size_t)z * ng + g;
+      if (P > cap) {
+        const int64_t nch = std::min<int64_t>((P + cap - 1) / cap, nx);
+        if (acc > 0) ++tiles;
+        tileOf[zg] = (int32_t)tiles;
+        chunksOf[zg] = (int32_t)nch;
+        tiles += nch;
+        acc = 0;
+        continue;
+      }
+      if (acc > 0 && acc + P > cap) {
+        ++tiles;
+        acc = 0;
+      }
+      tileOf[zg] = (int32_t)tiles;
+      chunksOf[zg] = 1;
+      acc += P;
+    }
+    if (acc > 0) ++tiles;
+    zBase[z + 1] = zBase[z] + tiles;
+  }
+  LIBP_ABORT("too many line tiles (" << zBase[Mz] << ")", zBase[Mz] > (int64_t)INT32_MAX);
+  std::vector<int32_t> key(Nrows);
+#pragma omp parallel for schedule(static)
+  for (dlong r = 0; r < Nrows; ++r) {
+    const hlong g = rowGrid[r];
+    const hlong i = g % Mx, j = (g / Mx) % My, k = g / (Mx * My);
+    const size_t zg = (size_t)k * ng + yg[j];
+    const hlong ex = std::min<hlong>(i / p, nx - 1);
+    key[r] = (int32_t)(zBase[k] + tileOf[zg] + (ex * chunksOf[zg]) / nx);
+  }
+  order = CountingOrder(Nrows, key, zBase[Mz]);
+  cuts.malloc(Nrows);
+#pragma omp parallel for schedule(static)
+  for (dlong i = 0; i < Nrows; ++i) cuts[i] = (i == 0) || key[order[i]] != key[order[i - 1]];
+}
+
+static memory<dlong> CountingOrder(const dlong Nrows, const std::vector<int32_t> &key, const hlong nkeys) {
+  memory<dlong> order(Nrows);
   // stable counting sort by block: each thread counts its contiguous share of
   // the rows, the counts are scanned block-major then thread-major, and each
   // thread places its rows in row order.  The shares follow the team the

@@ -98,6 +98,17 @@ SRB_CHUNK = np.dtype([("runMask", "<u8"), ("linkMask", "<u8"), ("runBase", "<i4"
                       ("chunkRow", "<i4"), ("nvalid", "<i4")])
 
 
+def hex_gs_tiles(nx, ny, nz, p, rows_per_line, cap, row_grid):
+    """Line-aligned gather+scatter tiles of a hex box (spxh_hex_gs_tiles): (order, cuts) of the rows
+    whose lattice points are row_grid."""
+    g = np.ascontiguousarray(row_grid, dtype=np.int64)
+    order = np.empty(g.size, dtype=np.int32)
+    cuts = np.empty(g.size, dtype=np.uint8)
+    checkh(libspxhost().spxh_hex_gs_tiles(nx, ny, nz, p, rows_per_line, cap, g.size, g.ctypes.data,
+                                          order.ctypes.data, cuts.ctypes.data), "spxh_hex_gs_tiles")
+    return order, cuts
+
+
 def srb_build(row_starts, col_ids, block_starts, cap, n_csr=None, ncols=None):
     """Sorted-run tiles (spx_srb_t) of a CSR and its row blocks, as host arrays.  n_csr = (rowStartsN,
     colIdsN) of a Signed operator whose T-CSR this is (ncols local nodes) adds its N-member masks

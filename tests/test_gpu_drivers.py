@@ -63,6 +63,20 @@ def test_gather_scatter_drivers(name):
     assert int(row(r.stdout, name).split("[")[1].split(",")[1]) == 51 * 46 * 41
 
 
+@pytest.mark.parametrize("name", ["BS6", "BS7", "BS8"])
+def test_synced_rows(name):
+    """BSnsync: a host finish after every call, as the reference's ogs_t waits inside each call; it
+    can only take longer per call than the kernel-only time."""
+    r = run(name, "-nx", "8", "-ny", "8", "-nz", "8", "-p", "7", "--settle", "0.05")
+    assert r.returncode == 0, r.stderr[-2000:]
+    sync = row(r.stdout, name + "sync")
+    vals = [float(x) for x in sync.split("[")[1].split("]")[0].split(",")]
+    dev = [float(x) for x in row(r.stdout, name + "device").split("[")[1].split("]")[0].split(",")]
+    assert int(vals[1]) == 57 ** 3 and vals[2] > 0
+    assert vals[2] >= dev[2]
+    assert "back-to-back" in sync
+
+
 def test_bs0_plain_and_graph():
     for extra in ([], ["--graph", "TRUE"]):
         r = run("BS0", "-n", "2000", *extra)

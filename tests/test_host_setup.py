@@ -302,3 +302,38 @@ def test_hex_box_setup_with_a_smaller_thread_team():
     env = dict(os.environ, OMP_NUM_THREADS="8", OMP_THREAD_LIMIT="3", PYTHONPATH=root)
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("box,r,cap", [((6, 5, 4, 7), 1, 2048), ((6, 5, 4, 7), 2, 2048), ((6, 5, 4, 7), 4, 4096),
+                                       ((9, 3, 2, 3), 2, 2048), ((4, 4, 4, 8), 2, 2048), ((20, 3, 3, 7), 2, 2048),
+                                       ((64, 3, 2, 7), 2, 2048)])
+def test_hex_line_tiles(box, r, cap):
+    """Line-aligned gather+scatter tiles: a permutation of the rows cut into tiles of at most cap
+    members, in which every group of r node rows of an element (one memory line of the vector) lies
+    in one tile, except where a group of y points too large for one tile is cut along x."""
+    nx, ny, nz, p = box
+    ids, c = ogs.build_csr_hex(nx, ny, nz, p, spx.SIGNED, True)
+    rs, ci = c["rowStartsT"], c["colIdsT"]
+    nrows = rs.size - 1
+    grid = np.abs(ids[ci[rs[:-1]]]) - 1
+    order, cuts = ogs.hex_gs_tiles(nx, ny, nz, p, r, cap, grid)
+    assert np.array_equal(np.sort(order), np.arange(nrows)) and cuts[0] == 1
+    tile_of_pos = np.cumsum(cuts) - 1                      # tile of each entry of order
+    tile_of_row = np.empty(nrows, dtype=np.int64)
+    tile_of_row[order] = tile_of_pos
+    sizes = np.bincount(tile_of_pos, weights=np.diff(rs)[order])
+    row_of = np.empty(ids.size, dtype=np.int64)
+    row_of[ci] = np.repeat(np.arange(nrows), np.diff(rs))
+    Nq = p + 1
+    t = tile_of_row[row_of].reshape(nx * ny * nz, Nq, Nq, Nq)         # element, k, j, i
+    split = 0
+    for j0 in range(0, Nq, r):
+        g = t[:, :, j0:j0 + r, :].reshape(nx * ny * nz, Nq, -1)
+        split += int((g != g[..., :1]).any(axis=-1).sum())
+    G = nx * p + 1
+    big = (nx * (p + 1)) * 2 * r * 2 > cap                   # some y group may exceed cap: x cuts allowed
+    assert sizes.max() <= cap or big
+    if not big:
+        assert split == 0
+    else:
+        assert 0 < split <= (nx * ny * nz * Nq * Nq // r) // 16   # only the element columns at x cuts
