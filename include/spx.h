@@ -168,6 +168,20 @@ int spx_reduce_2(int64_t nblocks, const double *partials, double *out, int block
 int spx_bs5_full(int64_t n, double alpha, const double *p, const double *Ap, double *x,
                  double *r, double *tmp, double *rdotr, int mode, int block_size);
 
+/* spx_bs5_full with no tmp argument and no allocation, for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+   The partials live in a library-owned array of SPX_NOALLOC_SCRATCH_DOUBLES
+   doubles per device (32 KiB, held until the process ends); a request whose
+   partial count (spx_reduce_tmp_count) exceeds it is refused with
+   SPX_ERR_ARG, which no block size or FAST grid of this library reaches.
+   Every call on a device shares that array: calls that overlap on one device
+   (two streams, two host threads) race.  Results are bit-identical to
+   spx_bs5_full's with the same mode and block size. */
+enum { SPX_NOALLOC_SCRATCH_DOUBLES = 4096 };
+int spx_bs5_full_noalloc(int64_t n, double alpha, const double *p, const double *Ap, double *x,
+                         double *r, double *rdotr, int mode, int block_size);
+
 /* ---------------------------------------------------------- ogs kernels -- */
 /* The CSR operator of ogsOperator_t (include/ogs/ogsOperator.hpp:44-70): rows
    are gathered nodes, rowStarts has nrows+1 entries, colIds holds local

@@ -119,6 +119,7 @@ SPX_FUNCS = {
     "spx_bs5_1": (C.c_int, [i64, i64, vp, dbl, vp, vp, C.c_int]),
     "spx_reduce_2": (C.c_int, [i64, vp, vp, C.c_int]),
     "spx_bs5_full": (C.c_int, [i64, dbl, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int]),
+    "spx_bs5_full_noalloc": (C.c_int, [i64, dbl, vp, vp, vp, vp, vp, C.c_int, C.c_int]),
     "spx_ogs_gather": (C.c_int, [C.c_int, C.c_int, C.c_int, i32, vp, vp, vp, i32, vp, vp]),
     "spx_ogs_scatter": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, i32, vp, vp]),
     "spx_ogs_gather_scatter": (C.c_int, [C.c_int, C.c_int, C.c_int, i32, vp, i32, vp, vp, i32, vp, vp, i32, vp]),

@@ -35,6 +35,10 @@ bool device_ready();
 hipStream_t current_stream();
 int *kernel_error_flag();   // device word kernels set on a violated precondition
 int num_cus();
+int device_id();            // the open device's HIP id, -1 when none is open
+// takes the device address of the allocation-free entry point's scratch array
+// (spx_stream.hip); called once by spx_device_open with the device current
+int scratch_setup(int device_id);
 
 #define SPX_NEED_DEVICE()                                                          \
   do {                                                                            \

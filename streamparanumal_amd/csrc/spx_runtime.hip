@@ -36,6 +36,7 @@ bool device_ready() { return g_dev.ready; }
 hipStream_t current_stream() { return g_dev.current; }
 int *kernel_error_flag() { return g_dev.err_flag; }
 int num_cus() { return g_dev.cus; }
+int device_id() { return g_dev.ready ? g_dev.id : -1; }
 
 static std::mutex g_tune_mu;
 static std::map<std::string, spx_tuning_t> g_tune;
@@ -197,6 +198,7 @@ int spx_device_open(int device_id) {
   SPX_HIP(hipMalloc((void **)&g_dev.err_flag, 256));
   SPX_HIP(hipMemset(g_dev.err_flag, 0, 256));
   SPX_HIP(hipDeviceSynchronize());
+  if (int rc = scratch_setup(device_id)) return rc;
   g_dev.current = g_dev.own_stream;
   g_dev.id = device_id;
   g_dev.ready = true;

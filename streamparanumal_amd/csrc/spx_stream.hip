@@ -417,6 +417,36 @@ static int run_reduction(const char *name, int64_t n, const RedArgs &A, double *
   return SPX_OK;
 }
 
+// ------------------------------------------------ allocation-free scratch ----
+// Partials of the allocation-free CG update (spx_bs5_full_noalloc): one
+// file-scope array per device, sized for the largest partial count any
+// reduction shape here produces (kFastGridMax; REFERENCE mode needs at most
+// its block size, <= 1024).  The host never passes `scratch` itself to a
+// kernel (in host code the name is only a stand-in for the device symbol):
+// spx_device_open takes its device address once, with the device current,
+// and every launch uses that address.  Stage 1 writes every partial that
+// stage 2 reads, so nothing has to start at zero; calls that overlap on one
+// device (several streams or host threads) share the array and race.
+constexpr int kScratchDoubles = kFastGridMax;
+static_assert(kScratchDoubles == SPX_NOALLOC_SCRATCH_DOUBLES, "spx.h states the scratch size");
+static_assert(kScratchDoubles >= 1024, "scratch must hold REFERENCE-mode partials");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchDevices = 64;
+static double *g_scratch_ptr[kScratchDevices] = {};
+
+namespace spx {
+int scratch_setup(int device_id) {
+  if (device_id < 0 || device_id >= kScratchDevices)
+    return fail(SPX_ERR_UNSUPPORTED, "scratch_setup: device id " + std::to_string(device_id) + " past the table");
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(SPX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device_id] = static_cast<double *>(ptr);
+  return SPX_OK;
+}
+}  // namespace spx
+
 // The full form's x/p address phase shift (RedArgs::xshift), in entries.
 // Default: half the vector, rounded down to 64 entries.  Measured on MI355X,
 // N = 2^27, the four vectors carved 1 GiB + d apart (TB/s, no shift / half):
@@ -595,6 +625,23 @@ int spx_bs5_full(int64_t n, double alpha, const double *p, const double *Ap, dou
   if (n > 0 && (!p || !Ap || !x || !r)) return fail(SPX_ERR_ARG, "spx_bs5_full: null input");
   RedArgs A{p, Ap, r, x, alpha, full_xshift(n), n};
   return run_reduction<RED_CGFULL>("spx_bs5_full", n, A, tmp, rdotr, mode, bs);
+}
+
+int spx_bs5_full_noalloc(int64_t n, double alpha, const double *p, const double *Ap, double *x, double *r,
+                         double *rdotr, int mode, int bs) {
+  SPX_NEED_DEVICE();
+  const int dev = device_id();
+  double *ptr = (dev >= 0 && dev < kScratchDevices) ? g_scratch_ptr[dev] : nullptr;
+  if (!ptr) return fail(SPX_ERR_NODEVICE, "spx_bs5_full_noalloc: scratch not set up for this device");
+  if (mode != SPX_REDUCE_REFERENCE && mode != SPX_REDUCE_FAST)
+    return fail(SPX_ERR_ARG, "spx_bs5_full_noalloc: unknown reduction mode");
+  const int64_t need = spx_reduce_tmp_count(n, mode, bs);
+  if (need < 0 || need > kScratchDoubles)
+    return fail(SPX_ERR_ARG, "spx_bs5_full_noalloc: " + std::to_string(need) + " partials exceed the " +
+                                 std::to_string(kScratchDoubles) + "-double scratch");
+  if (n > 0 && (!p || !Ap || !x || !r)) return fail(SPX_ERR_ARG, "spx_bs5_full_noalloc: null input");
+  RedArgs A{p, Ap, r, x, alpha, full_xshift(n), n};
+  return run_reduction<RED_CGFULL>("spx_bs5_full_noalloc", n, A, ptr, rdotr, mode, bs);
 }
 
 }  // extern "C"
