@@ -9,7 +9,11 @@ on MI355X, WRITE_SIZE all of it).
     python -m streamparanumal_amd.counters summarize <trace dir> <fetch dir> <write dir>
 
 prints the per-kernel table (durations from a ``--kernel-trace --stats`` run, counter medians and
-bytes) that the committed profile summaries quote.
+bytes) that the committed profile summaries quote;
+
+    python -m streamparanumal_amd.counters table <kernel regex> <pmc dir> [<pmc dir> ...]
+
+prints the median per dispatch of every counter of any --pmc passes for the matching kernels.
 """
 import csv
 import glob
@@ -155,9 +159,26 @@ def summarize(trace_dir, fetch_dir, write_dir, top=12):
     return "\n".join(lines)
 
 
+def counter_table(dirs, pattern="."):
+    """Markdown table: per kernel matching `pattern`, the median per dispatch of every counter found
+    in the --pmc passes under `dirs`."""
+    vals = {}
+    for d in dirs:
+        for (name, ctr), vs in counter_values(d).items():
+            if re.search(pattern, name):
+                vals.setdefault(name, {})[ctr] = statistics.median(vs)
+    ctrs = sorted({c for v in vals.values() for c in v})
+    lines = ["| kernel | " + " | ".join(ctrs) + " |", "|---|" + "---|" * len(ctrs)]
+    for name, v in sorted(vals.items()):
+        lines.append(f"| `{name[:90]}` | " + " | ".join(f"{v[c]:.4g}" if c in v else "-" for c in ctrs) + " |")
+    return "\n".join(lines)
+
+
 if __name__ == "__main__":
     if len(sys.argv) >= 5 and sys.argv[1] == "summarize":
         print(summarize(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]) if len(sys.argv) > 5 else 12))
+    elif len(sys.argv) >= 4 and sys.argv[1] == "table":
+        print(counter_table(sys.argv[3:], sys.argv[2]))
     else:
         print(__doc__)
         sys.exit(2)

@@ -199,10 +199,15 @@ __device__ __forceinline__ void pack(float (&o)[4], u32x2 &v) {
 
 constexpr int kCgGridMax = 2048;  // partials written by stage 1
 
+// x and p of chunk c + xs (mod the chunk count) are updated beside r and Ap
+// of chunk c, so the x / p streams run at another address phase than the
+// r / Ap streams (as spx_bs5_full); each x entry is still updated once from
+// its own p.
 template <typename S, int U, bool NT>
 __global__ __launch_bounds__(256) void k_cg_mixed1(int64_t n, float alpha, const S *__restrict__ p_,
                                                    const S *__restrict__ Ap_, S *__restrict__ x_,
-                                                   S *__restrict__ r_, double *__restrict__ partials) {
+                                                   S *__restrict__ r_, double *__restrict__ partials,
+                                                   int64_t xs) {
   typedef typename Quad<S>::type V;
   const V *__restrict__ p = reinterpret_cast<const V *>(p_);
   const V *__restrict__ Ap = reinterpret_cast<const V *>(Ap_);
@@ -214,13 +219,16 @@ __global__ __launch_bounds__(256) void k_cg_mixed1(int64_t n, float alpha, const
   const int64_t stride = (int64_t)256 * gridDim.x;
   int64_t c = t + (int64_t)blockIdx.x * 256;
   double acc = 0.0;
+  auto xc = [&](int64_t cc) { const int64_t j = cc + xs; return j >= nchunk ? j - nchunk : j; };
   for (; c + (int64_t)(U - 1) * stride < nchunk; c += (int64_t)U * stride) {
     V vp[U], vap[U], vx[U], vr[U];
+    int64_t cx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      vp[u] = ldq<NT>(p + c + u * stride);
+      cx[u] = xc(c + u * stride);
+      vp[u] = ldq<NT>(p + cx[u]);
       vap[u] = ldq<NT>(Ap + c + u * stride);
-      vx[u] = ldq<NT>(x + c + u * stride);
+      vx[u] = ldq<NT>(x + cx[u]);
       vr[u] = ldq<NT>(r + c + u * stride);
     }
 #pragma unroll
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(256) void k_cg_mixed1(int64_t n, float alpha, const
       }
       pack(fx, vx[u]);
       pack(fr, vr[u]);
-      stq<NT>(x + c + u * stride, vx[u]);
+      stq<NT>(x + cx[u], vx[u]);
       stq<NT>(r + c + u * stride, vr[u]);
 #pragma unroll
       for (int e = 0; e < 4; ++e) acc += (double)fr[e] * (double)fr[e];
@@ -247,9 +255,10 @@ __global__ __launch_bounds__(256) void k_cg_mixed1(int64_t n, float alpha, const
   }
   for (; c < nchunk; c += stride) {
     float fp[4], fap[4], fx[4], fr[4];
-    unpack(p[c], fp);
+    const int64_t cx = xc(c);
+    unpack(p[cx], fp);
     unpack(Ap[c], fap);
-    unpack(x[c], fx);
+    unpack(x[cx], fx);
     unpack(r[c], fr);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -261,7 +270,7 @@ __global__ __launch_bounds__(256) void k_cg_mixed1(int64_t n, float alpha, const
     V vx, vr;
     pack(fx, vx);
     pack(fr, vr);
-    x[c] = vx;
+    x[cx] = vx;
     r[c] = vr;
 #pragma unroll
     for (int e = 0; e < 4; ++e) acc += (double)fr[e] * (double)fr[e];
@@ -325,8 +334,11 @@ int launch_cg_mixed(int64_t n, float alpha, const S *p, const S *Ap, S *x, S *r,
   if (grid < 1) grid = 1;
   if (grid > kCgGridMax) grid = kCgGridMax;
   const dim3 g((unsigned)grid);
+  // x / p shift: half the chunks by default (variant v > 0: v chunks)
+  const int64_t nchunk = n / 4;
+  const int64_t xs = nchunk > 0 ? (tu.variant > 0 ? (int64_t)tu.variant : nchunk / 2) % nchunk : 0;
 #define SPX_CG(UU, NTT) \
-  hipLaunchKernelGGL((k_cg_mixed1<S, UU, NTT>), g, dim3(256), 0, current_stream(), n, alpha, p, Ap, x, r, tmp)
+  hipLaunchKernelGGL((k_cg_mixed1<S, UU, NTT>), g, dim3(256), 0, current_stream(), n, alpha, p, Ap, x, r, tmp, xs)
   if (U == 1) { if (nt) SPX_CG(1, true); else SPX_CG(1, false); }
   else if (U == 2) { if (nt) SPX_CG(2, true); else SPX_CG(2, false); }
   else if (U == 4) { if (nt) SPX_CG(4, true); else SPX_CG(4, false); }

@@ -230,9 +230,14 @@ LocalCSR BuildHexBoxCSR(const dlong nx, const dlong ny, const dlong nz, const in
 // Mx x My x Mz lattice.
 memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid, const hlong Mx, const hlong My,
                                 const hlong Mz, const int bx, const int by, const int bz);
+// Stable counting sort of the rows by key[r] in [0, nkeys).
+memory<dlong> CountingOrder(const dlong Nrows, const std::vector<int32_t> &key, const hlong nkeys);
 // Block shape measured best on MI355X for a hex box of degree p (64^3 p=7 and
 // ~1e8-node boxes at p = 1..15).
 void HexGsBlockShape(const int p, int &bx, int &by, int &bz);
+// Node rows per line tile a hex box of degree p uses for its gather+scatter
+// tiles (HexLineTiles), or 0 for the lattice blocks of HexGsBlockShape.
+int HexGsLineRows(const int p);
 // Gather+scatter tiles of an nx x ny x nz hex box at degree p whose members
 // fill whole memory lines of the vector: a line holds rowsPerLine node rows
 // (fixed j, k) of one element, so a tile takes every row of a full x line of

@@ -1,5 +1,6 @@
 // C ABI of the host layer (include/spx_host.h): ogs_t and mesh ids for
 // non-C++ callers.  Exceptions become SPX_ERR_* codes plus a message.
+#include "../../csrc/spx_hex.hpp"
 #include <string>
 
 #include "mesh.hpp"
@@ -250,15 +251,22 @@ int spxh_ogs_set_gs_tiles(spxh_ogs_t h, const int64_t *keys) {
       op->SetGsRowOrder(memory<dlong>());
       return;
     }
-    std::vector<std::pair<int64_t, dlong>> kr(n);
-    for (dlong r = 0; r < n; ++r) kr[r] = {keys[r], r};
-    std::sort(kr.begin(), kr.end());
-    memory<dlong> order(n);
-    memory<uint8_t> cuts(n);
-    for (dlong i = 0; i < n; ++i) {
-      order[i] = kr[i].second;
-      cuts[i] = (i == 0) || kr[i].first != kr[i - 1].first;
+    const int64_t kmin = *std::min_element(keys, keys + n), kmax = *std::max_element(keys, keys + n);
+    memory<dlong> order;
+    if (kmax - kmin < 4 * (int64_t)n + 1024) {
+      // keys in a dense range (tile numbers): stable counting sort
+      std::vector<int32_t> k(n);
+      for (dlong r = 0; r < n; ++r) k[r] = (int32_t)(keys[r] - kmin);
+      order = ogs::CountingOrder(n, k, kmax - kmin + 1);
+    } else {
+      std::vector<std::pair<int64_t, dlong>> kr(n);
+      for (dlong r = 0; r < n; ++r) kr[r] = {keys[r], r};
+      std::sort(kr.begin(), kr.end());
+      order.malloc(n);
+      for (dlong i = 0; i < n; ++i) order[i] = kr[i].second;
     }
+    memory<uint8_t> cuts(n);
+    for (dlong i = 0; i < n; ++i) cuts[i] = (i == 0) || keys[order[i]] != keys[order[i - 1]];
     op->SetGsRowOrder(order, false, cuts);
   });
 }
@@ -553,6 +561,42 @@ int spxh_hex_gs_tiles(int nx, int ny, int nz, int p, int rows_per_line, int cap,
     if (nrows > 0) {
       std::copy(o.ptr(), o.ptr() + nrows, order);
       std::copy(c.ptr(), c.ptr() + nrows, cuts);
+    }
+  });
+}
+
+int spxh_hex_tiles(int nx, int ny, int nz, int p, int rows_per_line, int cap, int32_t *ntiles, int32_t *tiles,
+                   int32_t max_tiles) {
+  if (!ntiles || nx < 1 || ny < 1 || nz < 1 || p < 1 || cap < 1) return err(SPX_ERR_ARG, "spxh_hex_tiles: bad arguments");
+  return guarded([&] {
+    const spx::hex::Box B{nx, ny, nz, p};
+    int32_t n = 0;
+    const bool ok = spx::hex::for_each_tile(B, rows_per_line, cap, [&](const spx::hex::Tile &t) {
+      if (tiles && n < max_tiles) {
+        tiles[4 * n] = t.z;
+        tiles[4 * n + 1] = t.y0;
+        tiles[4 * n + 2] = t.y1;
+        tiles[4 * n + 3] = t.npos;
+      }
+      ++n;
+    });
+    LIBP_ABORT("hex box too long in x for line tiles of " << cap << " positions", !ok);
+    *ntiles = n;
+  });
+}
+
+int spxh_hex_tile_nodes(int nx, int ny, int nz, int p, const int32_t *tile, int64_t *local, int32_t *prev) {
+  if (!tile || !local || !prev) return err(SPX_ERR_ARG, "spxh_hex_tile_nodes: null");
+  return guarded([&] {
+    const spx::hex::Box B{nx, ny, nz, p};
+    const spx::hex::Tile T{tile[0], tile[1], tile[2], tile[3]};
+    spx::hex::Shape S;
+    LIBP_ABORT("spxh_hex_tile_nodes: tile does not fit the decode", !spx::hex::tile_shape(B, T, S));
+    LIBP_ABORT("spxh_hex_tile_nodes: tile positions disagree", S.nz_slots * S.per_slot != T.npos);
+    for (int P = 0; P < T.npos; ++P) {
+      int pv;
+      spx::hex::tile_node(B, S, P, local[P], pv);
+      prev[P] = pv;
     }
   });
 }

@@ -137,8 +137,6 @@ void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order, const bool validat
   BuildGsSrb(gsSrb.xcdRemap, gsSrb.tileLanes);
 }
 
-// Stable counting sort of rows by key[r] in [0, nkeys).
-static memory<dlong> CountingOrder(const dlong Nrows, const std::vector<int32_t> &key, const hlong nkeys);
 
 memory<dlong> LatticeBlockOrder(const dlong Nrows, const memory<hlong> &rowGrid, const hlong Mx, const hlong My,
                                 const hlong Mz, const int bx, const int by, const int bz) {
@@ -225,7 +223,7 @@ void HexLineTiles(const dlong Nrows, const memory<hlong> &rowGrid, const hlong n
   for (dlong i = 0; i < Nrows; ++i) cuts[i] = (i == 0) || key[order[i]] != key[order[i - 1]];
 }
 
-static memory<dlong> CountingOrder(const dlong Nrows, const std::vector<int32_t> &key, const hlong nkeys) {
+memory<dlong> CountingOrder(const dlong Nrows, const std::vector<int32_t> &key, const hlong nkeys) {
   memory<dlong> order(Nrows);
   // stable counting sort by block: each thread counts its contiguous share of
   // the rows, the counts are scanned block-major then thread-major, and each
@@ -255,6 +253,17 @@ static memory<dlong> CountingOrder(const dlong Nrows, const std::vector<int32_t>
     for (dlong r = r0; r < r1; ++r) order[c[key[r]]++] = r;
   }
   return order;
+}
+
+int HexGsLineRows(const int p) {
+  // Measured on MI355X, us per call, setup's lattice blocks -> line tiles of
+  // 2 node rows (128-byte lines of FP64 values), FP64 / FP32 / BF16 storage:
+  // 64^3 p=7 481 / 351 / 364 -> 460 / 331 / 348 (three runs: 4-6 % for
+  // FP64), 58^3 p=7 365 -> 356; at other degrees the blocks were as fast or
+  // faster for FP64 (~1e8 nodes: p=1 483 -> 653, p=8 386 -> 443, p=9 370 ->
+  // 396, p=15 338 -> 344, p=3 409 -> 404; 1 or 4 rows per tile were no
+  // better), so only p=7 takes them.
+  return p == 7 ? 2 : 0;
 }
 
 void HexGsBlockShape(const int p, int &bx, int &by, int &bz) {

@@ -273,9 +273,9 @@ void ogsBase_t::SetupHexBox(const dlong nx, const dlong ny, const dlong nz, cons
   LocalCSR csr = BuildHexBoxCSR(nx, ny, nz, p, kind, unique, comm.rank(), idsOut);
   N = csr.nnzT;
   SetupFromCSR(csr);
-  // gather+scatter tiles over small 3-D blocks of grid points: a row's grid
-  // point from its first member (element e, node a + b*Nq + c*Nq^2, elements
-  // x fastest, meshSetupBoxHex3D.cpp:87-130)
+  // gather+scatter tiles over line tiles (HexGsLineRows) or small 3-D blocks
+  // of grid points: a row's grid point from its first member (element e, node
+  // a + b*Nq + c*Nq^2, elements x fastest, meshSetupBoxHex3D.cpp:87-130)
   {
     const ogsOperator_t &op = *gatherLocal;
     const hlong Nq = p + 1, Np = Nq * Nq * Nq;
@@ -287,9 +287,17 @@ void ogsBase_t::SetupHexBox(const dlong nx, const dlong ny, const dlong nz, cons
       const hlong ex = e % nx, ey = (e / nx) % ny, ez = e / ((hlong)nx * ny);
       grid[r] = (ex * p + n % Nq) + (ey * p + (n / Nq) % Nq) * Mx + (ez * p + n / (Nq * Nq)) * Mx * My;
     }
-    int bx, by, bz;
-    HexGsBlockShape(p, bx, by, bz);
-    gatherLocal->SetGsRowOrder(LatticeBlockOrder(op.NrowsT, grid, Mx, My, Mz, bx, by, bz), false);
+    const int rowsPerLine = HexGsLineRows(p);
+    if (rowsPerLine > 0) {
+      memory<dlong> order;
+      memory<uint8_t> cuts;
+      HexLineTiles(op.NrowsT, grid, nx, ny, nz, p, rowsPerLine, (int)op.gsBlockNnz, order, cuts);
+      gatherLocal->SetGsRowOrder(order, false, cuts);
+    } else {
+      int bx, by, bz;
+      HexGsBlockShape(p, bx, by, bz);
+      gatherLocal->SetGsRowOrder(LatticeBlockOrder(op.NrowsT, grid, Mx, My, Mz, bx, by, bz), false);
+    }
   }
   setupSeconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -619,3 +619,35 @@ def test_multi_field_gather_scatter_full_tiles(setups, dtype, k):
     g.gather_scatter(dq, k)
     device.check_kernel_errors()
     assert same(dq.to_numpy(), o.gather_scatter(q, k))
+
+
+@pytest.mark.parametrize("tiles", ["setup", "line2", "line1", "csr"])
+def test_hex_tile_orders(dev, oracle, tiles):
+    """A hex box's gather+scatter under each tile order (the setup's own, line tiles of 2 and 1 node
+    rows, the CSR's order), for FP64 and FP32 / BF16 storage: the oracle's bits (x extent of 11
+    elements: 78 lattice points, so line tiles are cut along x at cap 1024)."""
+    box = (11, 4, 3, 7)
+    say("hex tiles", tiles)
+    g = ogs.Ogs.hex(*box)
+    rs, ci, _ = g.csr("T")
+    if tiles != "setup":
+        g.set_block_nnz(g.info["block_nnz"], 1024)
+    if tiles.startswith("line"):
+        grid = np.abs(g.ids[ci[rs[:-1]]]) - 1
+        order, cuts = ogs.hex_gs_tiles(*box, int(tiles[-1]), 1024, grid)
+        key = np.empty(g.NlocalT, dtype=np.int64)
+        key[order] = np.cumsum(cuts) - 1
+        g.set_gs_tiles(key)
+    elif tiles == "csr":
+        g.set_gs_row_keys(None)
+    q = oracle.uniform(g.N, 80)
+    dq = DeviceArray.from_numpy(q)
+    g.gather_scatter(dq)
+    device.check_kernel_errors()
+    assert same(dq.to_numpy(), oracle.csr_gs(rs, ci, q))
+    for st in (spx.STORE_F32, spx.STORE_BF16):
+        qs = oracle.to_storage(q, st)
+        dq = DeviceArray.from_numpy(qs)
+        g.gather_scatter_mixed(dq)
+        device.check_kernel_errors()
+        assert same(dq.to_numpy(), oracle.gs_mixed(st, rs, ci, qs)), st
