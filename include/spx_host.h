@@ -166,18 +166,6 @@ int spxh_block_rows(int32_t nrows, const int32_t *rowStarts, int32_t cap, int32_
    boundaries. */
 int spxh_hex_gs_tiles(int nx, int ny, int nz, int p, int rows_per_line, int cap, int32_t nrows,
                       const int64_t *row_grid, int32_t *order, uint8_t *cuts);
-/* Line tiles of a hex box for the arithmetic decode of
-   spx_ogs_gather_scatter_hex (no reference counterpart): per tile 4 int32
-   {z plane, y0, y1, positions}, z plane by z plane, y ranges never splitting
-   rows_per_line node rows of an element, at most cap positions each.
-   *ntiles receives the count (tiles may be NULL to count only, or hold
-   max_tiles records).  Fails when the box is too long in x for cap. */
-int spxh_hex_tiles(int nx, int ny, int nz, int p, int rows_per_line, int cap, int32_t *ntiles, int32_t *tiles,
-                   int32_t max_tiles);
-/* The decode of one such tile on the host: local index of every position and
-   the position of the previous member of its point (-1: first member). */
-int spxh_hex_tile_nodes(int nx, int ny, int nz, int p, const int32_t *tile, int64_t *local, int32_t *prev);
-
 /* Sorted-run tiles (spx_srb_t) of a CSR and its row blocks, on the host (no
    device needed): sizes[0..5] = nblocks, nnz, nchunks (= nblocks *
    chunks_per_tile), nruns, nlinks, chunks_per_tile.  spxh_srb_copy fills

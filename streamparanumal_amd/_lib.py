@@ -160,8 +160,6 @@ SPXH_FUNCS = {
     "spxh_ogs_set_gs_row_keys": (C.c_int, [vp, vp]),
     "spxh_ogs_set_gs_tiles": (C.c_int, [vp, vp]),
     "spxh_hex_gs_tiles": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32, vp, vp, vp]),
-    "spxh_hex_tiles": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, i32]),
-    "spxh_hex_tile_nodes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
     "spxh_ogs_setup_batch": (C.c_int, [vp, i32, C.POINTER(vp)]),
     "spxh_ogs_setup_hex": (C.c_int, [i32, i32, i32, C.c_int, C.c_int, C.c_int, vp, C.POINTER(vp)]),
     "spxh_ogs_setup_box": (C.c_int, [C.c_int, i32, i32, i32, C.c_int, C.c_int, C.c_int, vp, C.POINTER(vp)]),

@@ -1,6 +1,5 @@
 // C ABI of the host layer (include/spx_host.h): ogs_t and mesh ids for
 // non-C++ callers.  Exceptions become SPX_ERR_* codes plus a message.
-#include "../../csrc/spx_hex.hpp"
 #include <string>
 
 #include "mesh.hpp"
@@ -561,42 +560,6 @@ int spxh_hex_gs_tiles(int nx, int ny, int nz, int p, int rows_per_line, int cap,
     if (nrows > 0) {
       std::copy(o.ptr(), o.ptr() + nrows, order);
       std::copy(c.ptr(), c.ptr() + nrows, cuts);
-    }
-  });
-}
-
-int spxh_hex_tiles(int nx, int ny, int nz, int p, int rows_per_line, int cap, int32_t *ntiles, int32_t *tiles,
-                   int32_t max_tiles) {
-  if (!ntiles || nx < 1 || ny < 1 || nz < 1 || p < 1 || cap < 1) return err(SPX_ERR_ARG, "spxh_hex_tiles: bad arguments");
-  return guarded([&] {
-    const spx::hex::Box B{nx, ny, nz, p};
-    int32_t n = 0;
-    const bool ok = spx::hex::for_each_tile(B, rows_per_line, cap, [&](const spx::hex::Tile &t) {
-      if (tiles && n < max_tiles) {
-        tiles[4 * n] = t.z;
-        tiles[4 * n + 1] = t.y0;
-        tiles[4 * n + 2] = t.y1;
-        tiles[4 * n + 3] = t.npos;
-      }
-      ++n;
-    });
-    LIBP_ABORT("hex box too long in x for line tiles of " << cap << " positions", !ok);
-    *ntiles = n;
-  });
-}
-
-int spxh_hex_tile_nodes(int nx, int ny, int nz, int p, const int32_t *tile, int64_t *local, int32_t *prev) {
-  if (!tile || !local || !prev) return err(SPX_ERR_ARG, "spxh_hex_tile_nodes: null");
-  return guarded([&] {
-    const spx::hex::Box B{nx, ny, nz, p};
-    const spx::hex::Tile T{tile[0], tile[1], tile[2], tile[3]};
-    spx::hex::Shape S;
-    LIBP_ABORT("spxh_hex_tile_nodes: tile does not fit the decode", !spx::hex::tile_shape(B, T, S));
-    LIBP_ABORT("spxh_hex_tile_nodes: tile positions disagree", S.nz_slots * S.per_slot != T.npos);
-    for (int P = 0; P < T.npos; ++P) {
-      int pv;
-      spx::hex::tile_node(B, S, P, local[P], pv);
-      prev[P] = pv;
     }
   });
 }

@@ -109,26 +109,6 @@ def hex_gs_tiles(nx, ny, nz, p, rows_per_line, cap, row_grid):
     return order, cuts
 
 
-def hex_tiles(nx, ny, nz, p, rows_per_line, cap):
-    """Line tiles of the arithmetic hex decode: int32 array (ntiles, 4) of {z, y0, y1, positions}."""
-    n = C.c_int32()
-    checkh(libspxhost().spxh_hex_tiles(nx, ny, nz, p, rows_per_line, cap, C.byref(n), None, 0), "spxh_hex_tiles")
-    t = np.zeros((n.value, 4), dtype=np.int32)
-    checkh(libspxhost().spxh_hex_tiles(nx, ny, nz, p, rows_per_line, cap, C.byref(n), t.ctypes.data, n.value),
-           "spxh_hex_tiles")
-    return t
-
-
-def hex_tile_nodes(nx, ny, nz, p, tile):
-    """Host decode of one line tile: (local index, previous member's position) per position."""
-    t = np.ascontiguousarray(tile, dtype=np.int32)
-    local = np.empty(int(t[3]), dtype=np.int64)
-    prev = np.empty(int(t[3]), dtype=np.int32)
-    checkh(libspxhost().spxh_hex_tile_nodes(nx, ny, nz, p, t.ctypes.data, local.ctypes.data, prev.ctypes.data),
-           "spxh_hex_tile_nodes")
-    return local, prev
-
-
 def srb_build(row_starts, col_ids, block_starts, cap, n_csr=None, ncols=None):
     """Sorted-run tiles (spx_srb_t) of a CSR and its row blocks, as host arrays.  n_csr = (rowStartsN,
     colIdsN) of a Signed operator whose T-CSR this is (ncols local nodes) adds its N-member masks

@@ -337,46 +337,3 @@ def test_hex_line_tiles(box, r, cap):
         assert split == 0
     else:
         assert 0 < split <= (nx * ny * nz * Nq * Nq // r) // 16   # only the element columns at x cuts
-
-
-@pytest.mark.parametrize("box,r,cap", [((3, 2, 2, 7), 2, 2048), ((4, 3, 3, 7), 2, 4096), ((5, 4, 3, 3), 2, 2048),
-                                       ((6, 2, 5, 1), 1, 1024), ((2, 5, 3, 4), 1, 1024), ((3, 3, 2, 15), 1, 4096),
-                                       ((1, 1, 1, 2), 1, 64), ((7, 9, 2, 2), 2, 2048)])
-def test_hex_tile_decode_matches_csr(box, r, cap):
-    """The arithmetic line-tile decode against the closed-form CSR: the tiles cover every local node once,
-    each tile's positions ascend in local index and hold every member of their points, and each position's
-    previous member is the previous column of its CSR row (-1 for the first)."""
-    nx, ny, nz, p = box
-    ids, c = ogs.build_csr_hex(nx, ny, nz, p, spx.SIGNED, True)
-    rs, ci = c["rowStartsT"], c["colIdsT"]
-    nrows = rs.size - 1
-    row_of = np.empty(ids.size, dtype=np.int64)
-    row_of[ci] = np.repeat(np.arange(nrows), np.diff(rs))
-    prev_col = np.full(ids.size, -1, dtype=np.int64)            # previous member of each local node's row
-    inner = np.ones(ci.size, dtype=bool)
-    inner[rs[:-1]] = False
-    prev_col[ci[inner]] = ci[np.nonzero(inner)[0] - 1]
-    tiles = ogs.hex_tiles(nx, ny, nz, p, r, cap)
-    assert np.all(tiles[:, 3] <= cap) and np.all(tiles[:, 3] > 0)
-    seen = np.zeros(ids.size, dtype=np.int64)
-    for t in tiles:
-        local, prev = ogs.hex_tile_nodes(nx, ny, nz, p, t)
-        assert np.all(np.diff(local) > 0)
-        seen[local] += 1
-        want = np.full(local.size, -1, dtype=np.int64)
-        pos_of = {int(l): i for i, l in enumerate(local)}
-        pc = prev_col[local]
-        has = pc >= 0
-        want[has] = [pos_of[int(x)] for x in pc[has]]           # KeyError: a member outside the tile
-        assert np.array_equal(prev, want)
-        rows = np.unique(row_of[local])
-        assert np.diff(rs)[rows].sum() == local.size          # whole rows only
-    assert np.all(seen == 1)
-
-
-def test_hex_tiles_refused():
-    """Line tiles that cannot exist: one x line above cap, or row groups wider than an element."""
-    with pytest.raises(SpxError):
-        ogs.hex_tiles(300, 2, 2, 7, 2, 2048)                     # 300 elements x 8 nodes x 2 rows > 2048
-    with pytest.raises(SpxError):
-        ogs.hex_tiles(7, 9, 2, 2, 4, 2048)                       # 4 rows per group at p=2: all of y at once
