@@ -36,6 +36,7 @@ typedef struct {
   int64_t NgatherGlobal;
   double setup_seconds;
   int32_t scatter_by_rows; /* scatter path: 1 row blocks, 0 local order (spxh_ogs_set_scatter_path) */
+  int32_t mixed_tiles[2];  /* FP32 / BF16 gather+scatter on tiles of their own (1) or the FP64 one's (0) */
 } spxh_ogs_info_t;
 
 const char *spxh_last_error(void);
@@ -102,6 +103,12 @@ int spxh_ogs_set_gs_row_keys(spxh_ogs_t ogs, const int64_t *keys);
    memory lines of the vector.  No reference counterpart; results are
    identical. */
 int spxh_ogs_set_gs_tiles(spxh_ogs_t ogs, const int64_t *keys);
+/* The same for the tiles of the FP32 (storage SPX_STORE_F32) or BF16
+   (SPX_STORE_BF16) gather+scatter (spxh_ogs_gather_scatter_mixed) alone;
+   keys NULL returns them to the tiles of the other types.  A hex box's setup
+   gives them line tiles of their own where those measured faster.  No
+   reference counterpart; results are identical. */
+int spxh_ogs_set_gs_mixed_tiles(spxh_ogs_t ogs, int storage, const int64_t *keys);
 /* layouts of the T-CSR gather (Trans) and scatter (NoTrans): 1 = the
    sorted-run blocks of the gather+scatter, 0 = T-CSR row blocks, -1 = keep;
    both sorted by default.  Results are identical. */

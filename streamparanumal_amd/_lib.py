@@ -55,7 +55,7 @@ class OgsInfo(C.Structure):
                 ("NblocksT", i32), ("NblocksN", i32), ("block_nnz", i32), ("gs_block_nnz", i32),
                 ("Ngs_blocks", i32), ("gather_defined", i32),
                 ("kind", i32), ("unique", i32), ("NgatherGlobal", i64), ("setup_seconds", dbl),
-                ("scatter_by_rows", i32)]
+                ("scatter_by_rows", i32), ("mixed_tiles", i32 * 2)]
 
 
 class GsTuning(C.Structure):
@@ -159,6 +159,7 @@ SPXH_FUNCS = {
     "spxh_ogs_set_gs_format": (C.c_int, [vp, C.c_int, C.c_int]),
     "spxh_ogs_set_gs_row_keys": (C.c_int, [vp, vp]),
     "spxh_ogs_set_gs_tiles": (C.c_int, [vp, vp]),
+    "spxh_ogs_set_gs_mixed_tiles": (C.c_int, [vp, C.c_int, vp]),
     "spxh_hex_gs_tiles": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32, vp, vp, vp]),
     "spxh_ogs_setup_batch": (C.c_int, [vp, i32, C.POINTER(vp)]),
     "spxh_ogs_setup_hex": (C.c_int, [i32, i32, i32, C.c_int, C.c_int, C.c_int, vp, C.POINTER(vp)]),

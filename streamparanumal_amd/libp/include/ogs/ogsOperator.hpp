@@ -133,6 +133,18 @@ class ogsOperator_t {
   // that each memory line of the vector is written by one tile.
   memory<uint8_t> gsTileCut;
   bool gsSorted = true;
+  // The FP32 and BF16 gather+scatter (GatherScatterMixedRaw) may take a tile
+  // order of their own: gsMixed[0] for SPX_STORE_F32, gsMixed[1] for
+  // SPX_STORE_BF16 (own = false: the gsSrb tiles).  The launch hints are
+  // gsSrb's.  A hex box's setup gives them line tiles where those measured
+  // faster for the narrower values (HexGsLineRowsMixed).
+  struct MixedTiles {
+    bool own = false;
+    memory<dlong> order;
+    memory<uint8_t> cuts;
+    SortedRunBlocks srb;
+  };
+  MixedTiles gsMixed[2];
   // The T-CSR gather (Trans) and scatter (NoTrans) on the same layout or on
   // the T-CSR row blocks of blockNnz entries.  Measured on MI355X at 64^3 hex
   // p=7 (algorithmic TB/s): scatter 6.6 sorted against 4.3 on the CSR,
@@ -183,6 +195,15 @@ class ogsOperator_t {
   void SetGsRowOrder(const memory<dlong> &order, bool validate = true, const memory<uint8_t> &cuts = memory<uint8_t>());
   // gsSrb from rowSrb and gsRowOrder, with these launch hints
   void BuildGsSrb(int xcdRemap, int tileLanes);
+  // Tiles of the FP32 (storage SPX_STORE_F32) or BF16 (SPX_STORE_BF16)
+  // gather+scatter in this row order with these cuts, as SetGsRowOrder; an
+  // empty order returns them to the gsSrb tiles.  Results do not change.
+  void SetGsMixedRowOrder(int storage, const memory<dlong> &order, const memory<uint8_t> &cuts,
+                          bool validate = true);
+  // The T-CSR's rows in `order` (empty: the CSR's), cut into greedy blocks of
+  // gsBlockNnz entries that also start where cuts[i] = 1, as sorted-run tiles
+  // uploaded to the device.
+  SortedRunBlocks BuildOrderedSrb(const memory<dlong> &order, const memory<uint8_t> &cuts);
 
   template <typename T> void Gather(deviceMemory<T> gv, const deviceMemory<T> v, const int k, const Op op, const Transpose trans);
   template <typename T> void Scatter(deviceMemory<T> v, const deviceMemory<T> gv, const int k, const Transpose trans);
@@ -238,6 +259,10 @@ void HexGsBlockShape(const int p, int &bx, int &by, int &bz);
 // Node rows per line tile a hex box of degree p uses for its gather+scatter
 // tiles (HexLineTiles), or 0 for the lattice blocks of HexGsBlockShape.
 int HexGsLineRows(const int p);
+// Node rows per line tile of the FP32 (storage SPX_STORE_F32) or BF16
+// (SPX_STORE_BF16) gather+scatter of a hex box of degree p, or 0 for the
+// tiles of the FP64 one.
+int HexGsLineRowsMixed(const int p, const int storage);
 // Gather+scatter tiles of an nx x ny x nz hex box at degree p whose members
 // fill whole memory lines of the vector: a line holds rowsPerLine node rows
 // (fixed j, k) of one element, so a tile takes every row of a full x line of

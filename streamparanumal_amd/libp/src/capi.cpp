@@ -161,6 +161,8 @@ int spxh_ogs_info(spxh_ogs_t h, spxh_ogs_info_t *info) {
     info->NgatherGlobal = o.NgatherGlobal;
     info->setup_seconds = o.setupSeconds();
     info->scatter_by_rows = op->scatterByRows ? 1 : 0;
+    info->mixed_tiles[0] = op->gsMixed[0].own ? 1 : 0;
+    info->mixed_tiles[1] = op->gsMixed[1].own ? 1 : 0;
   });
 }
 
@@ -240,6 +242,27 @@ int spxh_ogs_set_gs_row_keys(spxh_ogs_t h, const int64_t *keys) {
   });
 }
 
+namespace {
+// The n rows sorted by key (stable) and a tile cut wherever the key changes.
+void keys_to_tiles(const dlong n, const int64_t *keys, memory<dlong> &order, memory<uint8_t> &cuts) {
+  const int64_t kmin = *std::min_element(keys, keys + n), kmax = *std::max_element(keys, keys + n);
+  if (kmax - kmin < 4 * (int64_t)n + 1024) {
+    // keys in a dense range (tile numbers): stable counting sort
+    std::vector<int32_t> k(n);
+    for (dlong r = 0; r < n; ++r) k[r] = (int32_t)(keys[r] - kmin);
+    order = ogs::CountingOrder(n, k, kmax - kmin + 1);
+  } else {
+    std::vector<std::pair<int64_t, dlong>> kr(n);
+    for (dlong r = 0; r < n; ++r) kr[r] = {keys[r], r};
+    std::sort(kr.begin(), kr.end());
+    order.malloc(n);
+    for (dlong i = 0; i < n; ++i) order[i] = kr[i].second;
+  }
+  cuts.malloc(n);
+  for (dlong i = 0; i < n; ++i) cuts[i] = (i == 0) || keys[order[i]] != keys[order[i - 1]];
+}
+}  // namespace
+
 int spxh_ogs_set_gs_tiles(spxh_ogs_t h, const int64_t *keys) {
   if (!h || !keys) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_tiles: null");
   return guarded([&] {
@@ -250,23 +273,25 @@ int spxh_ogs_set_gs_tiles(spxh_ogs_t h, const int64_t *keys) {
       op->SetGsRowOrder(memory<dlong>());
       return;
     }
-    const int64_t kmin = *std::min_element(keys, keys + n), kmax = *std::max_element(keys, keys + n);
     memory<dlong> order;
-    if (kmax - kmin < 4 * (int64_t)n + 1024) {
-      // keys in a dense range (tile numbers): stable counting sort
-      std::vector<int32_t> k(n);
-      for (dlong r = 0; r < n; ++r) k[r] = (int32_t)(keys[r] - kmin);
-      order = ogs::CountingOrder(n, k, kmax - kmin + 1);
-    } else {
-      std::vector<std::pair<int64_t, dlong>> kr(n);
-      for (dlong r = 0; r < n; ++r) kr[r] = {keys[r], r};
-      std::sort(kr.begin(), kr.end());
-      order.malloc(n);
-      for (dlong i = 0; i < n; ++i) order[i] = kr[i].second;
-    }
-    memory<uint8_t> cuts(n);
-    for (dlong i = 0; i < n; ++i) cuts[i] = (i == 0) || keys[order[i]] != keys[order[i - 1]];
+    memory<uint8_t> cuts;
+    keys_to_tiles(n, keys, order, cuts);
     op->SetGsRowOrder(order, false, cuts);
+  });
+}
+
+int spxh_ogs_set_gs_mixed_tiles(spxh_ogs_t h, int storage, const int64_t *keys) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_mixed_tiles: null");
+  if (storage != SPX_STORE_F32 && storage != SPX_STORE_BF16)
+    return err(SPX_ERR_ARG, "spxh_ogs_set_gs_mixed_tiles: storage must be SPX_STORE_F32 or SPX_STORE_BF16");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    const dlong n = op->NrowsT;
+    memory<dlong> order;
+    memory<uint8_t> cuts;
+    if (keys && n > 0) keys_to_tiles(n, keys, order, cuts);
+    op->SetGsMixedRowOrder(storage, order, cuts, false);
   });
 }
 

@@ -59,6 +59,7 @@ GsTuning ogsOperator_t::Autotune(int type, int storage, int k, int reps, std::ve
   const memory<dlong> keepStarts = gsBlockRowStarts;
   const deviceMemory<dlong> keepO = o_gsBlockRowStarts;
   const SortedRunBlocks keepGs = gsSrb, keepRow = rowSrb;
+  const MixedTiles keepMixed[2] = {gsMixed[0], gsMixed[1]};
   GsTuning current{gsSorted ? 1 : 0, (int)gsBlockNnz, gsSrb.tileLanes, gsSrb.xcdRemap, 0.0};
   current.ms = timeCalls(run, reps);
   if (log) log->push_back(current);
@@ -90,6 +91,8 @@ GsTuning ogsOperator_t::Autotune(int type, int storage, int k, int reps, std::ve
     o_gsBlockRowStarts = keepO;
     gsSrb = keepGs;
     rowSrb = keepRow;
+    gsMixed[0] = keepMixed[0];
+    gsMixed[1] = keepMixed[1];
   } else if (best.blockNnz != gsBlockNnz) {
     SetGsBlockNnz(best.blockNnz);
   }

@@ -90,34 +90,72 @@ void ogsOperator_t::SetGsBlockNnz(int nnz) {
   rowSrb.tileLanes = rlanes;
   rowSrb.Upload(platform);
   BuildGsSrb(xcd, lanes);
+  // the FP32 / BF16 tiles at the new capacity (one layout where both share an order)
+  for (int s = 0; s < 2; ++s) {
+    MixedTiles &m = gsMixed[s];
+    if (!m.own) continue;
+    if (s == 1 && gsMixed[0].own && m.order.ptr() == gsMixed[0].order.ptr() && m.cuts.ptr() == gsMixed[0].cuts.ptr())
+      m.srb = gsMixed[0].srb;
+    else
+      m.srb = BuildOrderedSrb(m.order, m.cuts);
+  }
+}
+
+SortedRunBlocks ogsOperator_t::BuildOrderedSrb(const memory<dlong> &order, const memory<uint8_t> &cuts) {
+  const uint8_t *flags = signedSrb ? memberFlagN.ptr() : nullptr;
+  if (!order.length()) return rowSrb;
+  // the T-CSR with its rows in this order, cut into greedy blocks
+  memory<dlong> rs(NrowsT + 1), ci(nnzT > 0 ? nnzT : 1);
+#pragma omp parallel for schedule(static)
+  for (dlong i = 0; i < NrowsT; ++i) {
+    const dlong r = order[i];
+    rs[i] = rowStartsT[r + 1] - rowStartsT[r];
+  }
+  rs[NrowsT] = ExclusiveScan(rs.ptr(), NrowsT);
+#pragma omp parallel for schedule(static)
+  for (dlong i = 0; i < NrowsT; ++i) {
+    const dlong r = order[i];
+    std::copy(colIdsT.ptr() + rowStartsT[r], colIdsT.ptr() + rowStartsT[r + 1], ci.ptr() + rs[i]);
+  }
+  dlong nb = 0;
+  memory<dlong> bs;
+  blockRowsGreedy(NrowsT, rs, gsBlockNnz, nb, bs, cuts.length() ? cuts.ptr() : nullptr);
+  SortedRunBlocks srb = BuildSortedRunBlocks(NrowsT, rs, ci, nb, bs, gsBlockNnz, flags);
+  srb.Upload(platform);
+  return srb;
 }
 
 void ogsOperator_t::BuildGsSrb(int xcd, int lanes) {
-  const uint8_t *flags = signedSrb ? memberFlagN.ptr() : nullptr;
-  if (!gsRowOrder.length()) {
-    gsSrb = rowSrb;
-  } else {
-    // the T-CSR with its rows in gsRowOrder, cut into greedy blocks
-    memory<dlong> rs(NrowsT + 1), ci(nnzT > 0 ? nnzT : 1);
-#pragma omp parallel for schedule(static)
-    for (dlong i = 0; i < NrowsT; ++i) {
-      const dlong r = gsRowOrder[i];
-      rs[i] = rowStartsT[r + 1] - rowStartsT[r];
-    }
-    rs[NrowsT] = ExclusiveScan(rs.ptr(), NrowsT);
-#pragma omp parallel for schedule(static)
-    for (dlong i = 0; i < NrowsT; ++i) {
-      const dlong r = gsRowOrder[i];
-      std::copy(colIdsT.ptr() + rowStartsT[r], colIdsT.ptr() + rowStartsT[r + 1], ci.ptr() + rs[i]);
-    }
-    dlong nb = 0;
-    memory<dlong> bs;
-    blockRowsGreedy(NrowsT, rs, gsBlockNnz, nb, bs, gsTileCut.length() ? gsTileCut.ptr() : nullptr);
-    gsSrb = BuildSortedRunBlocks(NrowsT, rs, ci, nb, bs, gsBlockNnz, flags);
-    gsSrb.Upload(platform);
-  }
+  gsSrb = BuildOrderedSrb(gsRowOrder, gsTileCut);
   gsSrb.xcdRemap = xcd;
   gsSrb.tileLanes = lanes;
+}
+
+void ogsOperator_t::SetGsMixedRowOrder(const int storage, const memory<dlong> &order, const memory<uint8_t> &cuts,
+                                       const bool validate) {
+  LIBP_ABORT("mixed tiles are for SPX_STORE_F32 or SPX_STORE_BF16",
+             storage != SPX_STORE_F32 && storage != SPX_STORE_BF16);
+  MixedTiles &m = gsMixed[storage - 1];
+  if (!order.length()) {
+    m = MixedTiles();
+    return;
+  }
+  if (validate) {
+    LIBP_ABORT("gather+scatter row order must list the " << NrowsT << " rows", (dlong)order.length() != NrowsT);
+    std::vector<char> seen(NrowsT, 0);
+    for (dlong i = 0; i < NrowsT; ++i) {
+      const dlong r = order[i];
+      LIBP_ABORT("gather+scatter row order is not a permutation of the rows", r < 0 || r >= NrowsT || seen[r]);
+      seen[r] = 1;
+    }
+  }
+  LIBP_ABORT("gather+scatter tile cuts need one flag per row", cuts.length() && cuts.length() != order.length());
+  m.own = true;
+  m.order = order;
+  m.cuts = cuts;
+  const MixedTiles &o = gsMixed[2 - storage];  // the other storage's tiles
+  if (o.own && o.order.ptr() == order.ptr() && o.cuts.ptr() == cuts.ptr()) m.srb = o.srb;
+  else m.srb = BuildOrderedSrb(order, cuts);
 }
 
 void ogsOperator_t::SetGsRowOrder(const memory<dlong> &order, const bool validate, const memory<uint8_t> &cuts) {
@@ -266,6 +304,17 @@ int HexGsLineRows(const int p) {
   return p == 7 ? 2 : 0;
 }
 
+int HexGsLineRowsMixed(const int p, const int storage) {
+  // Measured on MI355X at ~1e8 nodes, us per call, the FP64 tiles -> line
+  // tiles: BF16 p=1 376 -> 253 (1 row), p=2 382 -> 342 (2 rows; 428 with 1),
+  // p=7 267 -> 254 (1 row; 64^3: 346 -> 329); FP32 p=7 247 -> 241 (64^3: 318
+  // -> 307).  FP32 lost at p=1 (363 -> 401) and p=2; at the other degrees
+  // either storage was within 2.5 % or slower.
+  if (storage == SPX_STORE_BF16) return p == 1 ? 1 : p == 2 ? 2 : p == 7 ? 1 : 0;
+  if (storage == SPX_STORE_F32) return p == 7 ? 1 : 0;
+  return 0;
+}
+
 void HexGsBlockShape(const int p, int &bx, int &by, int &bz) {
   // Measured on MI355X, us per call (CSR order -> blocks): ~1e8 nodes, FP64:
   // p=1 640 -> 436 (8x4x4), p=2 601 -> 447 (8x8x4), p=3 387 -> 361
@@ -393,7 +442,10 @@ void ogsOperator_t::GatherScatterRaw(Type type, Op op, Transpose trans, int k, v
 
 void ogsOperator_t::GatherScatterMixedRaw(int storage, int k, void *v) {
   if (NgsBlocks && gsSorted) {
-    const spx_srb_t d = gsSrb.Device();
+    const bool own = (storage == SPX_STORE_F32 || storage == SPX_STORE_BF16) && gsMixed[storage - 1].own;
+    spx_srb_t d = own ? gsMixed[storage - 1].srb.Device() : gsSrb.Device();
+    d.xcd_remap = gsSrb.xcdRemap;
+    d.tile_lanes = gsSrb.tileLanes;
     SPX_CHECK(spx_ogs_gather_scatter_srb_mixed(storage, k, &d, v));
   } else if (NgsBlocks)
     SPX_CHECK(spx_ogs_gather_scatter_mixed(storage, k, NgsBlocks, o_gsBlockRowStarts.ptr(), o_rowStartsT.ptr(),

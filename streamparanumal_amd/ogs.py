@@ -261,6 +261,7 @@ class Ogs:
         info = _lib.OgsInfo()
         checkh(libspxhost().spxh_ogs_info(self.h, C.byref(info)), "spxh_ogs_info")
         self.info = {f: getattr(info, f) for f, _ in _lib.OgsInfo._fields_}
+        self.info["mixed_tiles"] = list(self.info["mixed_tiles"])
         for k in ("N", "Ngather", "NlocalT", "NlocalP", "NgatherGlobal", "gather_defined"):
             setattr(self, k, self.info[k])
 
@@ -307,6 +308,19 @@ class Ogs:
         if k.size != self.NlocalT:
             raise ValueError(f"need {self.NlocalT} keys, got {k.size}")
         checkh(libspxhost().spxh_ogs_set_gs_tiles(self.h, k.ctypes.data), "spxh_ogs_set_gs_tiles")
+
+    def set_gs_mixed_tiles(self, storage, keys=None):
+        """As set_gs_tiles for the FP32 (storage STORE_F32) or BF16 (STORE_BF16) gather+scatter alone;
+        None returns it to the tiles of the other types.  Results do not change."""
+        if keys is None:
+            checkh(libspxhost().spxh_ogs_set_gs_mixed_tiles(self.h, storage, None), "spxh_ogs_set_gs_mixed_tiles")
+        else:
+            k = np.ascontiguousarray(keys, dtype=np.int64)
+            if k.size != self.NlocalT:
+                raise ValueError(f"need {self.NlocalT} keys, got {k.size}")
+            checkh(libspxhost().spxh_ogs_set_gs_mixed_tiles(self.h, storage, k.ctypes.data),
+                   "spxh_ogs_set_gs_mixed_tiles")
+        self.refresh()
 
     def set_gather_format(self, gather_sorted=None, scatter_sorted=None):
         """T-CSR gather (Trans) and scatter (NoTrans) on the sorted-run blocks (True) or the T-CSR row

@@ -79,6 +79,57 @@ def test_gs_mixed_f64_is_plain(setups, oracle):
     assert np.array_equal(dq.to_numpy().view(np.uint64), o.gather_scatter(q).view(np.uint64))
 
 
+# Hex boxes set up in closed form: FP32 / BF16 tiles of their own at p = 1, 2 (BF16) and 7 (both)
+HEX_OWN = {(6, 5, 4, 1): [0, 1], (5, 4, 3, 2): [0, 1], (12, 6, 3, 7): [1, 1], (4, 3, 3, 3): [0, 0]}
+
+
+@pytest.fixture(scope="module")
+def hex_setups(dev, oracle):
+    return {m: (ogs.Ogs.hex(*m), oracle.Ogs(ogs.hex_ids(*m))) for m in HEX_OWN}
+
+
+def gs_mixed_matches(g, o, oracle, st, k=1, seed=21):
+    q = oracle.to_storage(oracle.uniform(g.N * k, seed), st)
+    dq = DeviceArray.from_numpy(q)
+    g.gather_scatter_mixed(dq, k)
+    device.check_kernel_errors()
+    want = oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, q, k)
+    u = np.uint32 if st == STORE_F32 else np.uint16
+    return np.array_equal(dq.to_numpy().view(u), want.view(u))
+
+
+@pytest.mark.parametrize("mesh", list(HEX_OWN))
+@pytest.mark.parametrize("st", STORES)
+@pytest.mark.parametrize("k", [1, 3])
+def test_gs_mixed_own_tiles(hex_setups, oracle, mesh, st, k):
+    """The FP32 / BF16 tiles a hex box's setup gives by degree, bit-exact against the oracle."""
+    say("gs mixed own tiles", mesh, st, k)
+    g, o = hex_setups[mesh]
+    assert g.info["mixed_tiles"] == HEX_OWN[mesh]
+    assert gs_mixed_matches(g, o, oracle, st, k)
+
+
+@pytest.mark.parametrize("st", STORES)
+def test_gs_mixed_own_tiles_recut(dev, oracle, st):
+    """Own tiles survive a capacity change, an FP64 and a mixed autotune, a reset and custom keys."""
+    say("gs mixed own tiles recut", st)
+    m = (12, 6, 3, 7)
+    g, o = ogs.Ogs.hex(*m), oracle.Ogs(ogs.hex_ids(*m))
+    g.set_block_nnz(2048, gs_cap=512)
+    assert g.info["mixed_tiles"] == [1, 1] and gs_mixed_matches(g, o, oracle, st, seed=22)
+    g.set_block_nnz(2048, gs_cap=2048)
+    g.autotune(np.float64, reps=1)
+    g.autotune(NP[st], reps=1, mixed=True)
+    assert g.info["mixed_tiles"] == [1, 1] and gs_mixed_matches(g, o, oracle, st, seed=23)
+    g.set_gs_mixed_tiles(st, None)
+    assert g.info["mixed_tiles"][st - 1] == 0 and gs_mixed_matches(g, o, oracle, st, seed=24)
+    keys = np.random.default_rng(5).integers(0, 97, g.NlocalT)      # arbitrary row groups
+    g.set_gs_mixed_tiles(st, keys)
+    assert g.info["mixed_tiles"][st - 1] == 1 and gs_mixed_matches(g, o, oracle, st, 2, seed=25)
+    with pytest.raises(SpxError):
+        g.set_gs_mixed_tiles(STORE_F64, keys)
+
+
 def inputs(oracle, kind, n, seed):
     """The two input sets of the precision study: U[-1,1), and |x| log-uniform in [1e-6, 1] with random sign."""
     return oracle.uniform(n, seed) if kind == "uniform" else oracle.loguniform(n, seed)
