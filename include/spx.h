@@ -227,8 +227,10 @@ int spx_ogs_extract(int type, int K, int32_t n_ids, const int32_t *ids, const vo
    [b*chunks_per_tile, (b+1)*chunks_per_tile), record k describing positions
    64k..64k+63, of which the first nvalid are entries.  In record C:
      runs:  bit l of C.runMask is set when position l starts a run of
-            consecutive local indices (bit 0 always); the runs begin at local
-            indices runIds[C.runBase], runIds[C.runBase+1], ...
+            consecutive local indices (bit 0 always); run r of the record
+            holds runIds[C.runBase + r] + l at its positions l, i.e. the
+            entry is the run's first local index less the run's first
+            position
      rows:  bit l of C.linkMask is set when position l is not the first
             (smallest) member of its row; such positions, in order, have the
             tile position of the previous member of their row in

@@ -247,7 +247,8 @@ def to_storage(v, storage):
     """Round float64 values to the storage type (float32 array, or bf16 bits as uint16)."""
     v = np.ascontiguousarray(v, dtype=np.float64)
     if storage == STORE_F32:
-        return v.astype(np.float32)
+        with np.errstate(over="ignore"):  # past the f32 range: infinity, as the device's conversion
+            return v.astype(np.float32)
     if storage == STORE_BF16:
         f = lib().oracle_bf16_from_double
         return np.array([f(float(x)) for x in v], dtype=np.uint16) if v.size < 4096 else _bf16_vec(v)
@@ -256,15 +257,19 @@ def to_storage(v, storage):
 
 def _bf16_vec(v):
     # exact nearest-even rounding, vectorised (same rule as oracle_bf16_from_double)
-    f = v.astype(np.float32)
-    lo = f.view(np.uint32) & np.uint32(0xFFFF0000)
-    up = np.abs(lo.view(np.float32).astype(np.float64)) > np.abs(v)
-    lo = np.where(up, lo - np.uint32(0x10000), lo).astype(np.uint32)
-    hi = (lo + np.uint32(0x10000)).astype(np.uint32)
-    dl = np.abs(v - lo.view(np.float32).astype(np.float64))
-    dh = np.abs(hi.view(np.float32).astype(np.float64) - v)
-    r = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where((lo >> 16) & 1, hi, lo)))
-    return (r >> 16).astype(np.uint16)
+    with np.errstate(over="ignore", invalid="ignore"):
+        f = v.astype(np.float32)
+        lo = f.view(np.uint32) & np.uint32(0xFFFF0000)
+        up = np.abs(lo.view(np.float32).astype(np.float64)) > np.abs(v)
+        lo = np.where(up, lo - np.uint32(0x10000), lo).astype(np.uint32)
+        hi = (lo + np.uint32(0x10000)).astype(np.uint32)
+        # above the largest bf16 the next candidate is 2^128 in magnitude (infinity when taken)
+        hv = np.where((hi & np.uint32(0x7FFFFFFF)) == np.uint32(0x7F800000), np.copysign(2.0**128, v),
+                      hi.view(np.float32).astype(np.float64))
+        dl = np.abs(v - lo.view(np.float32).astype(np.float64))
+        dh = np.abs(hv - v)
+        r = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where((lo >> 16) & 1, hi, lo)))
+        return np.where(np.isnan(v), np.uint16(0x7FC0), (r >> 16).astype(np.uint16)).astype(np.uint16)
 
 
 def from_storage(q, storage):

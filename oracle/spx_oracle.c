@@ -753,7 +753,10 @@ uint16_t oracle_bf16_from_double(double x) {
   float fl, fh;
   memcpy(&fl, &lo, 4);
   memcpy(&fh, &hi, 4);
-  const double dl = fabs(x - (double)fl), dh = fabs((double)fh - x);
+  /* above the largest bf16 the next candidate is 2^128 in magnitude, i.e.
+     infinity when x rounds to it (round to nearest, IEEE 754 7.4) */
+  const double fhv = ((hi & 0x7fffffffu) == 0x7f800000u) ? copysign(0x1p128, x) : (double)fh;
+  const double dl = fabs(x - (double)fl), dh = fabs(fhv - x);
   uint32_t r;
   if (dl < dh) r = lo;
   else if (dh < dl) r = hi;

@@ -114,9 +114,7 @@ int launch_gs_srb_mixed(const Launch &L, const SrbArgs &A, S *q) {
   const bool wave = L.variant == 2 && L.cap <= 16 * 64;
   const int lanes = wave ? 64 : kThreads;
   // all fields per pass if they fit the registers and LDS, else half (K even), else one
-  auto lds_of = [&](int kf) {
-    return (size_t)W.cap * srb_lds_fields<S>(kf) * sizeof(S) + (size_t)W.cap * sizeof(unsigned short);
-  };
+  auto lds_of = [&](int kf) { return srb_gs_lds_bytes<S>(W.cap, kf); };
   auto fits = [&](int kf) {
     W.kf = kf;
     W.per = pick_per(kf, W.cap * (kThreads / lanes));

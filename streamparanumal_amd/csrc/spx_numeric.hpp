@@ -12,29 +12,30 @@ struct bf16_t {
   uint16_t bits;
 };
 
+// NaN of any sign and payload -> 0x7fc0, as torch's bfloat16 conversion.
+constexpr uint16_t kBf16NaN = 0x7fc0;
+
 // f64 -> bf16, round to nearest even, without the double rounding of a
 // f64->f32->bf16 chain: round to f32 with round-to-odd (exact when the f32
-// keeps 16 spare bits beyond bf16), then round-to-nearest-even to bf16.
+// keeps 16 spare bits beyond bf16), then round-to-nearest-even to bf16.  A
+// value past the f32 range rounds to f32 infinity, whose magnitude exceeds
+// it, so round-to-odd takes the largest finite f32 instead, and that rounds
+// to bf16 infinity (it lies above the midpoint between the largest bf16 and
+// 2^128), as the exact value does.
 __device__ __forceinline__ uint16_t f64_to_bf16_bits(double x) {
   const float f = (float)x;
   uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u)  // inf, nan
-    return (uint16_t)((u >> 16) | ((u & 0x007fffffu) ? 0x0040u : 0u));
   const double back = (double)f;
-  if (back != x) {
-    if (fabs(back) > fabs(x)) u -= 1u;  // truncate toward zero (sign-magnitude)
-    u |= 1u;                            // sticky bit: round to odd
-  }
+  u -= fabs(back) > fabs(x) ? 1u : 0u;  // truncate toward zero (sign-magnitude)
+  u |= back != x ? 1u : 0u;             // sticky bit: round to odd
   u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  return x != x ? kBf16NaN : (uint16_t)(u >> 16);
 }
 
 // f32 -> bf16, round to nearest even
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0x007fffffu) ? 0x0040u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  const uint32_t u = __float_as_uint(f);
+  return f != f ? kBf16NaN : (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
 __device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }

@@ -62,16 +62,15 @@ int launch_gs_sym(const Launch &L, int op, const int *bs, const int *st, const i
 // lanes.  All K fields go in one pass when the lanes' registers (pick_per)
 // and the LDS hold them; otherwise half of them per pass (K even), else one
 // per pass.  L.cap is the layout's positions per tile (chunks per tile * 64).
-// gs: the gather+scatter kernel, which keeps srb_lds_fields<T>(kf) fields of
-// a position in LDS (the gather and scatter kernels keep one).
+// gs: the gather+scatter kernel, whose LDS is srb_gs_lds_bytes (the gather
+// and scatter kernels keep one value and one u16 link per position).
 template <bool WITH_OP, typename F>
 static int srb_shape(const Launch &L, int op, F &&f, bool gs = false) {
   Launch W = L;
   const bool wave = L.variant == 2 && L.cap <= 16 * 64;
   const int lanes = wave ? 64 : kThreads;
   auto lds_of = [&](int kf) {
-    const int lf = gs ? srb_lds_fields<T>(kf) : 1;
-    return lds_bytes(W.cap, lf, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short);
+    return gs ? srb_gs_lds_bytes<T>(W.cap, kf) : lds_bytes(W.cap, 1, sizeof(T), 1) + (size_t)W.cap * sizeof(unsigned short);
   };
   auto fits = [&](int kf) {
     W.kf = kf;

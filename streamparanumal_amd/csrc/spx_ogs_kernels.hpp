@@ -222,8 +222,8 @@ __global__ __launch_bounds__(kThreads) void k_gs_sym(Launch L, const int *__rest
 // ("positions") and owns the chunk records [b*cpt, (b+1)*cpt), record k for
 // positions 64k..64k+63, the first nvalid of which are entries.  Position
 // l of record C decodes as
-//   local index = runIds[C.runBase + popc(C.runMask & bits<=l) - 1] + (l - top)
-//     where top is the highest set bit of C.runMask at or below l;
+//   local index = runIds[C.runBase + popc((C.runMask >> 1) & bits<l)] + l
+//     (each run's entry is its first local index less its first position);
 //   previous member of its row (bit l of C.linkMask set)
 //              = links[C.linkBase + popc(C.linkMask & bits<l)];
 //   row of a row-leading position = C.chunkRow + (l - popc(C.linkMask & bits<l)).
@@ -283,21 +283,24 @@ __device__ __forceinline__ void srb_decode(const SrbChunk *__restrict__ chunks, 
     const int k = (NT / 64) * j + w;  // wave-uniform
     const int src = (j & 7) * 8;
     const int wv = word[j / 8];
-    const unsigned long long rm = (unsigned)__builtin_amdgcn_readlane(wv, src + 0) |
-                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(wv, src + 1) << 32);
-    const unsigned long long lm = (unsigned)__builtin_amdgcn_readlane(wv, src + 2) |
-                                  ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(wv, src + 3) << 32);
+    // the record's fields, wave-uniform: the masks act on lane l as position l
+    const unsigned rmLo = __builtin_amdgcn_readlane(wv, src + 0), rmHi = __builtin_amdgcn_readlane(wv, src + 1);
+    const unsigned lmLo = __builtin_amdgcn_readlane(wv, src + 2), lmHi = __builtin_amdgcn_readlane(wv, src + 3);
     const int runBase = __builtin_amdgcn_readlane(wv, src + 4);
     const int linkBase = __builtin_amdgcn_readlane(wv, src + 5);
     const int chunkRow = __builtin_amdgcn_readlane(wv, src + 6);
-    const int nv = k < cpt ? min(__builtin_amdgcn_readlane(wv, src + 7), 64) : 0;
-    const bool ok = l < nv;
-    const int ls = ok ? l : 0;  // bit 0 of runMask is always set
-    const unsigned long long upto = (2ull << ls) - 1ull;
-    const unsigned long long rbel = rm & upto;
-    id[j] = runIds[runBase + __builtin_popcountll(rbel) - 1] + (ls - (63 - __builtin_clzll(rbel)));
-    const int before = __builtin_popcountll(lm & (upto >> 1));
-    const bool linked = ok && ((lm >> ls) & 1ull);
+    const int nraw = min(__builtin_amdgcn_readlane(wv, src + 7), 64);
+    const bool ok = k < cpt && l < nraw;
+    // run of position l: the run starts at positions 1..l (runMask >> 1 below
+    // lane l); a lane past the entries takes the last entry, so its address
+    // stays inside the run table and the vector
+    const unsigned long long rs = ((unsigned long long)rmHi << 32 | rmLo) >> 1;
+    const int run = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(rs >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)rs, 0u));
+    const int ls = min(l, max(nraw - 1, 0));
+    id[j] = runIds[runBase + run] + ls;
+    const unsigned long long lm = (unsigned long long)lmHi << 32 | lmLo;
+    const int before = (int)__builtin_amdgcn_mbcnt_hi(lmHi, __builtin_amdgcn_mbcnt_lo(lmLo, 0u));
+    const bool linked = ok && __builtin_amdgcn_inverse_ballot_w64(lm);
     prev[j] = linked ? (int)links[linkBase + before] : -1;
     row[j] = chunkRow + (ls - before);
     valid |= (ok ? 1u : 0u) << j;
@@ -343,6 +346,52 @@ __device__ __forceinline__ unsigned srb_members(const unsigned long long *__rest
 // field for 2-byte values (word-major, below), one field at a time otherwise.
 template <typename T> constexpr int srb_lds_fields(int kf) { return (kf > 1 && sizeof(T) == 2) ? kf : 1; }
 
+// LDS of a sorted-run gather+scatter tile of cap positions.  A 2- or 4-byte
+// value folded one field at a time shares an entry with its position's next
+// link (4 bytes: value, u16 link; 8 bytes: value, u32 link), so a step of a
+// row walk is one LDS access at one address; 8-byte values and the
+// word-major fields keep a value array and a u16 link array.
+template <typename T> constexpr bool srb_gs_packed(int kf) { return srb_lds_fields<T>(kf) == 1 && sizeof(T) <= 4; }
+template <typename T> constexpr size_t srb_gs_lds_bytes(int cap, int kf) {
+  return srb_gs_packed<T>(kf) ? (size_t)cap * (sizeof(T) == 2 ? 4 : 8)
+                              : (size_t)cap * srb_lds_fields<T>(kf) * sizeof(T) + (size_t)cap * sizeof(unsigned short);
+}
+
+template <typename T, int LF, bool PACK> struct GsLds;
+template <typename T, int LF> struct GsLds<T, LF, false> {
+  T *vals;
+  unsigned short *next;
+  __device__ GsLds(unsigned char *raw, int cap)
+      : vals(reinterpret_cast<T *>(raw)), next(reinterpret_cast<unsigned short *>(vals + (size_t)cap * LF)) {}
+  __device__ void set_next(int i, int n) const { next[i] = (unsigned short)n; }
+  __device__ int get_next(int i) const { return next[i]; }
+  __device__ void set_val(int i, int f, T x) const { vals[i * LF + f] = x; }
+  __device__ T get_val(int i, int f) const { return vals[i * LF + f]; }
+  __device__ T get_both(int i, int f, int &n) const {
+    n = next[i];
+    return vals[i * LF + f];
+  }
+};
+template <typename T> struct GsLds<T, 1, true> {
+  static_assert(sizeof(T) == 2 || sizeof(T) == 4, "packed entries hold 2- or 4-byte values");
+  typedef typename std::conditional<sizeof(T) == 2, unsigned short, unsigned>::type Link;
+  struct alignas(2 * sizeof(T)) Entry {
+    T val;
+    Link next;
+  };
+  Entry *e;
+  __device__ GsLds(unsigned char *raw, int) : e(reinterpret_cast<Entry *>(raw)) {}
+  __device__ void set_next(int i, int n) const { e[i].next = (Link)n; }
+  __device__ int get_next(int i) const { return (int)e[i].next; }
+  __device__ void set_val(int i, int, T x) const { e[i].val = x; }
+  __device__ T get_val(int i, int) const { return e[i].val; }
+  __device__ T get_both(int i, int, int &n) const {
+    const Entry x = e[i];  // one access
+    n = (int)x.next;
+    return x.val;
+  }
+};
+
 template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>, int MODE = kAll>
 __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restrict__ q,
                                                const unsigned long long *__restrict__ memberN) {
@@ -369,16 +418,21 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
                 "N-CSR forms fold plain 4- and 8-byte values");
   constexpr int LF = kWord ? KF : 1;  // fields held in LDS per position
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  T *vals = reinterpret_cast<T *>(lds_raw);
-  unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap * LF);
+  const int cap = L.cap;  // cpt * 64 positions
+  const GsLds<T, LF, srb_gs_packed<T>(KF)> S(lds_raw, cap);
+  T *vals = reinterpret_cast<T *>(lds_raw);  // word-major fields (kWord)
   const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
   const int kb = blockIdx.y * KF;
-  const int cap = L.cap;  // cpt * 64 positions
   const int64_t K = L.K;
   const int t = threadIdx.x;
+  if (cap >= PER * NT) {  // every slot a position (the usual tile)
 #pragma unroll
-  for (int j = 0; j < PER; ++j)
-    if (t + j * NT < cap) next[t + j * NT] = kEnd;
+    for (int j = 0; j < PER; ++j) S.set_next(t + j * NT, kEnd);
+  } else {
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (t + j * NT < cap) S.set_next(t + j * NT, kEnd);
+  }
   int id[PER], prev[PER], row[PER];
   unsigned valid;
   srb_decode<PER, NT>(chunks, runIds, links, b, cap / 64, id, prev, row, valid);
@@ -407,10 +461,10 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
         v[j][f] = q[(int64_t)idw * K + kb + (w - src * KF)];
       }
     }
-  __syncthreads();  // next[] initialised before the links go in
+  __syncthreads();  // links initialised before they are set
 #pragma unroll
   for (int j = 0; j < PER; ++j)
-    if (((valid >> j) & 1u) && prev[j] >= 0 && prev[j] < cap) next[prev[j]] = (unsigned short)(t + j * NT);
+    if (((valid >> j) & 1u) && (unsigned)prev[j] < (unsigned)cap) S.set_next(prev[j], t + j * NT);
   // one LDS round per field (one round in all for KF == 1 and word-major)
   constexpr int ROUNDS = kWord ? 1 : KF;
 #pragma unroll
@@ -419,7 +473,7 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       if constexpr (!kWord) {
-        if ((valid >> j) & 1u) vals[t + j * NT] = v[j][f0];
+        if ((valid >> j) & 1u) S.set_val(t + j * NT, 0, v[j][f0]);
       } else {
         const unsigned long long vm = __ballot((valid >> j) & 1u);
 #pragma unroll
@@ -438,22 +492,29 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
         for (int f = 0; f < LF; ++f) {
           typename F::Acc acc = F::init();
           // walks end at kEnd (> cap); the step bound also ends them on a malformed layout
-          for (int i = h, s = 0; i < cap && s < cap; i = next[i], ++s) F::apply(acc, vals[i * LF + f]);
+          for (int i = h, s = 0; i < cap && s < cap; ++s) {
+            int n;
+            F::apply(acc, S.get_both(i, f, n));
+            i = n;
+          }
           const T r = F::out(acc);
-          for (int i = h, s = 0; i < cap && s < cap; i = next[i], ++s) vals[i * LF + f] = r;
+          for (int i = h, s = 0; i < cap && s < cap; ++s) {
+            S.set_val(i, f, r);
+            i = S.get_next(i);
+          }
         }
       }
     __syncthreads();
     if constexpr (!kWord && KF > 1) {
 #pragma unroll
       for (int j = 0; j < PER; ++j)
-        if ((valid >> j) & 1u) v[j][f0] = vals[t + j * NT];
+        if ((valid >> j) & 1u) v[j][f0] = S.get_val(t + j * NT, 0);
     }
   }
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     if constexpr (KF == 1) {
-      if ((st >> j) & 1u) q[(int64_t)id[j] * K + kb] = vals[t + j * NT];
+      if ((st >> j) & 1u) q[(int64_t)id[j] * K + kb] = S.get_val(t + j * NT, 0);
     } else if constexpr (!kWord) {
       if ((st >> j) & 1u)
 #pragma unroll

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "spx_internal.hpp"
+#include "spx_numeric.hpp"
 
 namespace spx {
 
@@ -130,26 +131,10 @@ __global__ void k_f32_to_f64(int64_t n, const float *s, double *d) {
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += st) d[i] = (double)s[i];
 }
-// f64 -> bf16, round to nearest even, without the double rounding of a
-// f64->f32->bf16 chain: round to f32 with round-to-odd (exact when the f32
-// keeps 16 spare bits beyond bf16), then round-to-nearest-even to bf16.
-__device__ __forceinline__ uint16_t f64_to_bf16_rne(double x) {
-  const float f = (float)x;
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u)  // inf, nan
-    return (uint16_t)((u >> 16) | ((u & 0x007fffffu) ? 0x0040u : 0u));
-  const double back = (double)f;
-  if (back != x) {
-    if (fabs(back) > fabs(x)) u -= 1u;  // truncate toward zero (sign-magnitude)
-    u |= 1u;                            // sticky bit: round to odd
-  }
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
 __global__ void k_f64_to_bf16(int64_t n, const double *s, uint16_t *d) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += st) d[i] = f64_to_bf16_rne(s[i]);
+  for (; i < n; i += st) d[i] = f64_to_bf16_bits(s[i]);
 }
 __global__ void k_bf16_to_f64(int64_t n, const uint16_t *s, double *d) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

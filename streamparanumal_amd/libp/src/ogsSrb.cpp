@@ -129,7 +129,9 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
       C.linkBase = (int32_t)kl;
       for (dlong l = 0; l < C.nvalid; ++l) {
         const dlong i = e0 + 64 * k + l;
-        if (C.runMask >> l & 1ull) S.runIds[kr++] = sorted[i];
+        // a run's local index less its position in the chunk: position l of
+        // the run decodes as runIds[run] + l
+        if (C.runMask >> l & 1ull) S.runIds[kr++] = sorted[i] - l;
         if (C.linkMask >> l & 1ull) S.links[kl++] = (uint16_t)prevPos[i];
       }
     }

@@ -209,12 +209,16 @@ def test_device_conversions(dev, oracle, st):
     say("convert", st)
     n = 200003
     u = oracle.uniform(n, 41) * np.exp2((oracle.rand_int(n, 42) % 40) - 20.0)
-    u[:4] = [0.0, -0.0, 1.0 + 2.0**-8 + 2.0**-40, 1.0 + 2.0**-8]
+    big = (2 - 2.0**-8) * 2.0**127                  # bf16 overflow midpoint (ties to infinity)
+    special = [0.0, -0.0, 1.0 + 2.0**-8 + 2.0**-40, 1.0 + 2.0**-8, np.inf, -np.inf, np.nan, -np.nan, big, -big,
+               np.nextafter(big, 0), 1e39, -1e300, 3.0e38, 2.0**-133, 2.0**-134, 1.5 * 2.0**-134, 1e-45, -1e-46]
+    u[:len(special)] = special
     src = DeviceArray.from_numpy(u)
     dst = DeviceArray(n, NP[st])
     device.convert(src, dst)
     want = oracle.to_storage(u, st)
-    assert np.array_equal(dst.to_numpy(), want)
+    bits = np.uint32 if st == STORE_F32 else np.uint16
+    assert np.array_equal(dst.to_numpy().view(bits), want.view(bits))
     back = DeviceArray(n)
     device.convert(dst, back)
-    assert np.array_equal(back.to_numpy(), oracle.from_storage(want, st))
+    assert np.array_equal(back.to_numpy().view(np.uint64), oracle.from_storage(want, st).view(np.uint64))

@@ -96,10 +96,8 @@ def decode_srb(s):
             C = s["chunks"][b * cpt + k]
             rm, lm = int(C["runMask"]), int(C["linkMask"])
             for l in range(int(C["nvalid"])):
-                below = rm & ((2 << l) - 1)
-                run = bin(below).count("1") - 1
-                top = below.bit_length() - 1
-                idx = int(s["runIds"][C["runBase"] + run]) + (l - top)
+                run = bin(rm & ((2 << l) - 1)).count("1") - 1
+                idx = int(s["runIds"][C["runBase"] + run]) + l
                 before = bin(lm & ((1 << l) - 1)).count("1")
                 prev = int(s["links"][C["linkBase"] + before]) if lm >> l & 1 else -1
                 row = int(C["chunkRow"]) + (l - before) if prev < 0 else None

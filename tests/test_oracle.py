@@ -378,6 +378,23 @@ def test_bf16_rounding_known_answers(oracle):
     assert f(1.0 + 2.0**-8 - 2.0**-40) == 0x3F80
     assert f(float("inf")) == 0x7F80 and f(-float("inf")) == 0xFF80
     assert f(3.0e38) == 0x7F62                   # 3e38 -> 1.1015625 * 2^127 (nearest bf16)
+    big = (2 - 2.0**-8) * 2.0**127              # midpoint of the largest bf16 and 2^128: ties to even (inf)
+    assert f(big) == 0x7F80 and f(-big) == 0xFF80
+    assert f(np.nextafter(big, 0)) == 0x7F7F and f(1e39) == 0x7F80 and f(-1e300) == 0xFF80
+    assert f(float("nan")) == 0x7FC0 and f(-float("nan")) == 0x7FC0
+    assert f(2.0**-133) == 0x0001 and f(2.0**-134) == 0 and f(1.5 * 2.0**-134) == 0x0001   # denormals
+
+
+def test_bf16_vectorised_matches_scalar(oracle):
+    """The vectorised rounding (large arrays) against the scalar C rule, specials included."""
+    f = oracle.lib().oracle_bf16_from_double
+    big = (2 - 2.0**-8) * 2.0**127
+    v = np.concatenate([oracle.uniform(5000, 8) * np.exp2((oracle.rand_int(5000, 9) % 300) - 150.0),
+                        [0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, big, -big, np.nextafter(big, 0), 1e39,
+                         -1e300, 2.0**-133, 2.0**-134, 1.5 * 2.0**-134, 3.0e38, 1.0 + 2.0**-8 + 2.0**-40]])
+    want = np.array([f(float(x)) for x in v], dtype=np.uint16)
+    got = oracle.to_storage(np.concatenate([v] * 2), oracle.STORE_BF16)[: v.size]   # >= 4096: vectorised path
+    assert np.array_equal(got, want)
 
 
 def test_bf16_from_float_matches_torch(oracle):
