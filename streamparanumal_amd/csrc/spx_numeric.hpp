@@ -28,14 +28,14 @@ __device__ __forceinline__ uint16_t f64_to_bf16_bits(double x) {
   const double back = (double)f;
   u -= fabs(back) > fabs(x) ? 1u : 0u;  // truncate toward zero (sign-magnitude)
   u |= back != x ? 1u : 0u;             // sticky bit: round to odd
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return x != x ? kBf16NaN : (uint16_t)(u >> 16);
+  // nearest even to bf16: one v_cvt_pk_bf16_f32 on gfx950
+  const uint16_t r = __builtin_bit_cast(uint16_t, (__bf16)__uint_as_float(u));
+  return x != x ? kBf16NaN : r;
 }
 
 // f32 -> bf16, round to nearest even
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return f != f ? kBf16NaN : (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  return f != f ? kBf16NaN : __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 
 __device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
