@@ -488,19 +488,34 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
     for (int j = 0; j < PER; ++j)
       if (((valid >> j) & 1u) && prev[j] < 0) {
         const int h = t + j * NT;
-#pragma unroll
-        for (int f = 0; f < LF; ++f) {
+        if constexpr (KF == 1) {
+          // One walk: each member's link, once read, is replaced by the head's
+          // position, and the head keeps the row value; after the barrier
+          // every member takes the value from its head in one step instead
+          // of a second walk writing it along the row.
           typename F::Acc acc = F::init();
           // walks end at kEnd (> cap); the step bound also ends them on a malformed layout
           for (int i = h, s = 0; i < cap && s < cap; ++s) {
             int n;
-            F::apply(acc, S.get_both(i, f, n));
+            F::apply(acc, S.get_both(i, 0, n));
+            S.set_next(i, h);
             i = n;
           }
-          const T r = F::out(acc);
-          for (int i = h, s = 0; i < cap && s < cap; ++s) {
-            S.set_val(i, f, r);
-            i = S.get_next(i);
+          S.set_val(h, 0, F::out(acc));
+        } else {
+#pragma unroll
+          for (int f = 0; f < LF; ++f) {
+            typename F::Acc acc = F::init();
+            for (int i = h, s = 0; i < cap && s < cap; ++s) {
+              int n;
+              F::apply(acc, S.get_both(i, f, n));
+              i = n;
+            }
+            const T r = F::out(acc);
+            for (int i = h, s = 0; i < cap && s < cap; ++s) {
+              S.set_val(i, f, r);
+              i = S.get_next(i);
+            }
           }
         }
       }
@@ -514,7 +529,12 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     if constexpr (KF == 1) {
-      if ((st >> j) & 1u) q[(int64_t)id[j] * K + kb] = S.get_val(t + j * NT, 0);
+      // the row value from the head (a position never walked, only on a
+      // malformed layout, keeps its own)
+      if ((st >> j) & 1u) {
+        const int hd = S.get_next(t + j * NT);
+        q[(int64_t)id[j] * K + kb] = S.get_val((unsigned)hd < (unsigned)cap ? hd : t + j * NT, 0);
+      }
     } else if constexpr (!kWord) {
       if ((st >> j) & 1u)
 #pragma unroll
