@@ -381,15 +381,19 @@ template <typename T> struct GsLds<T, 1, true> {
   };
   Entry *e;
   __device__ GsLds(unsigned char *raw, int) : e(reinterpret_cast<Entry *>(raw)) {}
+  // the next link is the low 16 bits; a 32-bit link word can also carry the
+  // row's head position in its high 16 bits (set_link_head / get_head)
   __device__ void set_next(int i, int n) const { e[i].next = (Link)n; }
-  __device__ int get_next(int i) const { return (int)e[i].next; }
+  __device__ int get_next(int i) const { return (int)(e[i].next & 0xffffu); }
   __device__ void set_val(int i, int, T x) const { e[i].val = x; }
   __device__ T get_val(int i, int) const { return e[i].val; }
   __device__ T get_both(int i, int, int &n) const {
     const Entry x = e[i];  // one access
-    n = (int)x.next;
+    n = (int)(x.next & 0xffffu);
     return x.val;
   }
+  __device__ void set_link_head(int i, int n, int h) const { e[i].next = (Link)((unsigned)n | ((unsigned)h << 16)); }
+  __device__ int get_head(int i) const { return (int)((unsigned)e[i].next >> 16); }
 };
 
 template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>, int MODE = kAll>
@@ -417,6 +421,8 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
   static_assert(MODE == kAll || (!kWord && std::is_same<typename F::Acc, T>::value),
                 "N-CSR forms fold plain 4- and 8-byte values");
   constexpr int LF = kWord ? KF : 1;  // fields held in LDS per position
+  // several 4-byte fields: heads kept beside the links (one walk per field)
+  constexpr bool kHeadLink = KF > 1 && srb_gs_packed<T>(KF) && sizeof(T) == 4;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int cap = L.cap;  // cpt * 64 positions
   const GsLds<T, LF, srb_gs_packed<T>(KF)> S(lds_raw, cap);
@@ -502,6 +508,18 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
             i = n;
           }
           S.set_val(h, 0, F::out(acc));
+        } else if constexpr (kHeadLink) {
+          // several fields, one per round: the first round's walk also
+          // writes the head's position beside each member's link, and every
+          // round's members read the row value from the head
+          typename F::Acc acc = F::init();
+          for (int i = h, s = 0; i < cap && s < cap; ++s) {
+            int n;
+            F::apply(acc, S.get_both(i, 0, n));
+            if (f0 == 0) S.set_link_head(i, n, h);
+            i = n;
+          }
+          S.set_val(h, 0, F::out(acc));
         } else {
 #pragma unroll
           for (int f = 0; f < LF; ++f) {
@@ -520,7 +538,14 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
         }
       }
     __syncthreads();
-    if constexpr (!kWord && KF > 1) {
+    if constexpr (kHeadLink) {
+#pragma unroll
+      for (int j = 0; j < PER; ++j)
+        if ((valid >> j) & 1u) {
+          const int hd = S.get_head(t + j * NT);
+          v[j][f0] = S.get_val((unsigned)hd < (unsigned)cap ? hd : t + j * NT, 0);
+        }
+    } else if constexpr (!kWord && KF > 1) {
 #pragma unroll
       for (int j = 0; j < PER; ++j)
         if ((valid >> j) & 1u) v[j][f0] = S.get_val(t + j * NT, 0);
