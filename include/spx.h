@@ -248,7 +248,9 @@ typedef struct {
 typedef struct {
   int32_t nblocks;
   int32_t chunks_per_tile; /* 1..64: positions per tile = 64 * chunks_per_tile <= 4096 */
-  int32_t xcd_remap;       /* 1: consecutive tiles on one XCD, 0: hardware order, -1: library default */
+  int32_t xcd_remap;       /* 1: consecutive tiles on one XCD in blocks (128 tiles; 32 for the FP32 /
+                              BF16 gather+scatter), a power of two 2..1024: blocks of that many
+                              tiles, 0: hardware order, negative: library default */
   int32_t tile_lanes;      /* lanes per tile: 64 (one wave, <= 1024 positions) or 256; 0: library default */
   const spx_srb_chunk_t *chunks;
   const int32_t *runIds;

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kThreads) void k_gs_mixed(Launch L, const int *__re
   extern __shared__ __align__(16) unsigned char lds_raw[];
   S *vals = reinterpret_cast<S *>(lds_raw);
   unsigned short *rs = reinterpret_cast<unsigned short *>(lds_raw + (size_t)L.cap * KF * sizeof(S));
-  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd, L.xcdLog2);
   const int kb = blockIdx.y * KF;
   const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
   const int nrows = r1 - r0;
@@ -403,7 +403,9 @@ int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, v
   Launch L;
   L.nblocks = srb->nblocks;
   L.K = K;
-  L.nxcd = (srb->xcd_remap >= 0 ? srb->xcd_remap : tu.xcd_remap) ? 8 : 1;
+  L.xcdLog2 = kXcdLog2Mixed;
+  if (!xcd_hint(srb->xcd_remap, tu.xcd_remap, L.nxcd, L.xcdLog2))
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: xcd_remap must be -1, 0, 1 or a power of two up to 1024");
   L.cap = 64 * srb->chunks_per_tile;
   L.kf = (K >= 2 && K <= 4) ? K : 1;
   L.per = 0;

@@ -42,6 +42,7 @@ struct Launch {
   int nblocks;
   int K;      // fields per node (stride)
   int nxcd;   // XCD remap width (1 = identity)
+  int xcdLog2 = kXcdLog2;  // XCD remap: blocks of 2^xcdLog2 consecutive tiles
   int cap;    // LDS capacity in entries (>= every block's nnz)
   int kf;     // fields per pass
   int per;    // entries per lane (cap <= 256*per)
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(kThreads) void k_gs_sym(Launch L, const int *__rest
   T *vals = reinterpret_cast<T *>(lds_raw);
   int *rs = reinterpret_cast<int *>(vals + (size_t)L.cap * KF);
   unsigned short *rs16 = reinterpret_cast<unsigned short *>(rs);
-  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd, L.xcdLog2);
   const int kb = blockIdx.y * KF;
   const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
   const int nrows = r1 - r0;
@@ -427,7 +428,7 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
   const int cap = L.cap;  // cpt * 64 positions
   const GsLds<T, LF, srb_gs_packed<T>(KF)> S(lds_raw, cap);
   T *vals = reinterpret_cast<T *>(lds_raw);  // word-major fields (kWord)
-  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd, L.xcdLog2);
   const int kb = blockIdx.y * KF;
   const int64_t K = L.K;
   const int t = threadIdx.x;
@@ -591,7 +592,7 @@ __global__ __launch_bounds__(NT) void k_gather_srb(Launch L, SRB_PARAMS, const T
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
   unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap);
-  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd, L.xcdLog2);
   const int kb = blockIdx.y * KF;
   const int cap = L.cap;
   const int64_t K = L.K;
@@ -652,7 +653,7 @@ __global__ __launch_bounds__(NT) void k_scatter_srb(Launch L, SRB_PARAMS, const 
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
   unsigned short *next = reinterpret_cast<unsigned short *>(vals + (size_t)L.cap);
-  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd, L.xcdLog2);
   const int kb = blockIdx.y * KF;
   const int cap = L.cap;
   const int64_t K = L.K;
@@ -715,7 +716,7 @@ __global__ __launch_bounds__(kThreads) void k_gs_gen(Launch L, int ngrows, int n
   T *svals = gvals + (size_t)L.cap * KF;
   int *grs = reinterpret_cast<int *>(svals + (size_t)L.cap * KF);
   int *srs = grs + L.cap + 1;
-  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd, L.xcdLog2);
   const int kb = blockIdx.y * KF;
   const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
   const int nrows = r1 - r0;
@@ -783,7 +784,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(Launch L, const int *__rest
                                                      const T *__restrict__ v, T *__restrict__ gv) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
-  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd, L.xcdLog2);
   const int kb = blockIdx.y * KF;
   const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
   const int nrows = r1 - r0;
@@ -835,7 +836,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(Launch L, const int *__res
                                                       const T *__restrict__ gv, T *__restrict__ v) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *vals = reinterpret_cast<T *>(lds_raw);
-  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd);
+  const int b = xcd_tile(blockIdx.x, L.nblocks, L.nxcd, L.xcdLog2);
   const int kb = blockIdx.y * KF;
   const int r0 = blockStarts[b], r1 = blockStarts[b + 1];
   const int nrows = r1 - r0;

@@ -70,6 +70,26 @@ def test_gs_mixed_block_caps(setups, oracle, cap, st):
         g.set_block_nnz(2048, gs_cap=2048)
 
 
+@pytest.mark.parametrize("st", STORES)
+def test_gs_mixed_xcd_hints(setups, oracle, st):
+    """Every XCD remap hint of the mixed kernel gives the same bits; other values are refused."""
+    say("gs mixed xcd", st)
+    g, o = setups[(8, 8, 8, 7)]
+    q = oracle.to_storage(oracle.uniform(g.N, 14), st)
+    want = oracle.gs_mixed(st, o.rowStartsT, o.colIdsT, q)
+    try:
+        for xcd in (-1, 0, 1, 2, 32, 1024):
+            g.set_gs_format(True, xcd)
+            dq = DeviceArray.from_numpy(q)
+            g.gather_scatter_mixed(dq)
+            device.check_kernel_errors()
+            assert np.array_equal(dq.to_numpy(), want)
+        with pytest.raises(SpxError):
+            g.set_gs_format(True, 5)
+    finally:
+        g.set_gs_format(True, -1)
+
+
 def test_gs_mixed_f64_is_plain(setups, oracle):
     say("gs mixed f64")
     g, o = setups[(5, 3, 2, 4)]

@@ -171,7 +171,7 @@ def test_result_independent_of_row_blocks(setups, oracle, cap, variant):
 @pytest.mark.parametrize("mesh", MESHES)
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("variant,cap,xcd", [(0, 1024, 0), (0, 4096, 1), (2, 1024, 1), (2, 4096, 0),
-                                             (2, 512, 0), (2, 256, 1)])
+                                             (2, 512, 0), (2, 256, 1), (0, 2048, 32), (2, 1024, 2)])
 def test_srb_equals_csr_format(setups, oracle, mesh, dtype, variant, cap, xcd):
     """The sorted-run kernels (256-lane and one-wave tiles) and the T-CSR blocks give the same bits,
     for every op and K = 1, 3."""
@@ -252,12 +252,18 @@ def test_srb_direct_random_ids(dev, oracle, seed, cap):
     _, c = ogs.build_csr(ids)
     rs, ci = c["rowStartsT"], c["colIdsT"]
     bs = ogs.block_rows(rs, cap)
-    d = ogs.SrbDevice(ogs.srb_build(rs, ci, bs, cap))
+    srb = ogs.srb_build(rs, ci, bs, cap)
     q = oracle.uniform(n, seed)
-    dq = DeviceArray.from_numpy(q)
-    d.gather_scatter(dq)
-    device.check_kernel_errors()
-    assert same(dq.to_numpy(), oracle.csr_gs(rs, ci, q))
+    want = oracle.csr_gs(rs, ci, q)
+    for xcd in (-1, -2, 0, 1, 4, 1024):             # XCD remap hints: default (any negative), off, on, blocks
+        d = ogs.SrbDevice(srb, xcd_remap=xcd)
+        dq = DeviceArray.from_numpy(q)
+        d.gather_scatter(dq)
+        device.check_kernel_errors()
+        assert same(dq.to_numpy(), want)
+    for bad in (3, 2048):
+        with pytest.raises(SpxError):
+            ogs.SrbDevice(srb, xcd_remap=bad).gather_scatter(DeviceArray.from_numpy(q))
     # gather then scatter on the layout reproduces the gather+scatter
     nrows = rs.size - 1
     dg = DeviceArray(nrows)
