@@ -87,7 +87,7 @@ int spxh_ogs_set_gs_block_nnz(spxh_ogs_t ogs, int32_t block_nnz);
 int spxh_ogs_set_scatter_path(spxh_ogs_t ogs, int by_rows);
 /* layout of the Sym gather+scatter: 1 = sorted-run row blocks (spx_srb_t, the
    default), 0 = the T-CSR row blocks; results are identical.  xcd_remap is the
-   sorted-run launch hint (-1 library default, 0 hardware order, 1 remap in
+   sorted-run launch hint (negative: library default, 0 hardware order, 1 remap in
    the kernel's default blocks, a power of two 2..1024: blocks of that many
    tiles; as spx_srb_t.xcd_remap). */
 int spxh_ogs_set_gs_format(spxh_ogs_t ogs, int sorted, int xcd_remap);

@@ -405,7 +405,7 @@ int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, v
   L.K = K;
   L.xcdLog2 = kXcdLog2Mixed;
   if (!xcd_hint(srb->xcd_remap, tu.xcd_remap, L.nxcd, L.xcdLog2))
-    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: xcd_remap must be -1, 0, 1 or a power of two up to 1024");
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: xcd_remap must be negative (library default), 0, 1 or a power of two up to 1024");
   L.cap = 64 * srb->chunks_per_tile;
   L.kf = (K >= 2 && K <= 4) ? K : 1;
   L.per = 0;

@@ -166,7 +166,7 @@ int srb_launch_setup(const char *name, int type, int K, const spx_srb_t *srb, La
     return fail(SPX_ERR_ARG, std::string(name) + ": tile_lanes must be 0, 64 or 256");
   if ((rc = make_launch(name, "ogs_gs_srb", srb->nblocks, K, cap, type, 1, L))) return rc;
   if (srb->xcd_remap >= 0 && !xcd_hint(srb->xcd_remap, L.nxcd > 1, L.nxcd, L.xcdLog2))
-    return fail(SPX_ERR_ARG, std::string(name) + ": xcd_remap must be -1, 0, 1 or a power of two up to 1024");
+    return fail(SPX_ERR_ARG, std::string(name) + ": xcd_remap must be negative (library default), 0, 1 or a power of two up to 1024");
   if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
   A.chunks = reinterpret_cast<const SrbChunk *>(srb->chunks);
   A.runIds = srb->runIds;

@@ -214,8 +214,8 @@ int spxh_ogs_set_scatter_path(spxh_ogs_t h, int by_rows) {
 
 int spxh_ogs_set_gs_format(spxh_ogs_t h, int sorted, int xcd_remap) {
   if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_format: null");
-  if (xcd_remap < -1 || xcd_remap > 1024 || (xcd_remap > 1 && (xcd_remap & (xcd_remap - 1))))
-    return err(SPX_ERR_ARG, "spxh_ogs_set_gs_format: xcd_remap must be -1, 0, 1 or a power of two up to 1024");
+  if (xcd_remap > 1024 || (xcd_remap > 1 && (xcd_remap & (xcd_remap - 1))))
+    return err(SPX_ERR_ARG, "spxh_ogs_set_gs_format: xcd_remap must be negative (library default), 0, 1 or a power of two up to 1024");
   return guarded([&] {
     auto op = h->ogs.localOperator();
     LIBP_ABORT("ogs not set up", !op);
