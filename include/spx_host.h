@@ -37,6 +37,7 @@ typedef struct {
   double setup_seconds;
   int32_t scatter_by_rows; /* scatter path: 1 row blocks, 0 local order (spxh_ogs_set_scatter_path) */
   int32_t mixed_tiles[2];  /* FP32 / BF16 gather+scatter on tiles of their own (1) or the FP64 one's (0) */
+  int32_t gs_nontemporal;  /* spx_srb_t.nontemporal of the Sym gather+scatter's tiles */
 } spxh_ogs_info_t;
 
 const char *spxh_last_error(void);
@@ -111,6 +112,12 @@ int spxh_ogs_set_gs_tiles(spxh_ogs_t ogs, const int64_t *keys);
    gives them line tiles of their own where those measured faster.  No
    reference counterpart; results are identical. */
 int spxh_ogs_set_gs_mixed_tiles(spxh_ogs_t ogs, int storage, const int64_t *keys);
+/* Non-temporal value loads and stores of the Sym gather+scatter
+   (spx_srb_t.nontemporal): -1 as the tiles' builder set them (node sizes
+   whose 128-byte lines mostly lie whole in one tile), 0 never, 1 for every
+   node size; kept when the tiles are rebuilt.  No reference counterpart;
+   results are identical. */
+int spxh_ogs_set_gs_nontemporal(spxh_ogs_t ogs, int mode);
 /* layouts of the T-CSR gather (Trans) and scatter (NoTrans): 1 = the
    sorted-run blocks of the gather+scatter, 0 = T-CSR row blocks, -1 = keep;
    both sorted by default.  Results are identical. */
@@ -203,6 +210,10 @@ int spxh_srb_build_signed(int32_t nrows, const int32_t *rowStarts, const int32_t
                           const int32_t *blockStarts, int32_t max_block_nnz, int32_t ncols, int32_t nrowsN,
                           const int32_t *rowStartsN, const int32_t *colIdsN, spxh_srb_t *out);
 int spxh_srb_copy_members(spxh_srb_t srb, uint64_t *memberN);
+/* spx_srb_t.nontemporal for the layout: bit e (e = 1..5) when at least 85 %
+   of the (tile, 128-byte line) pairs of a vector of 2^e-byte nodes that hold
+   an entry hold the whole line. */
+int spxh_srb_nontemporal(spxh_srb_t srb, int32_t *mask);
 
 /* mesh_t hex box ids: lexicographic GLL-grid ids of an nx x ny x nz box at
    degree p, n = nx*ny*nz*(p+1)^3 entries (meshSetupBoxHex3D.cpp:87-130) */

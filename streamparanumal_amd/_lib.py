@@ -55,7 +55,8 @@ class OgsInfo(C.Structure):
                 ("NblocksT", i32), ("NblocksN", i32), ("block_nnz", i32), ("gs_block_nnz", i32),
                 ("Ngs_blocks", i32), ("gather_defined", i32),
                 ("kind", i32), ("unique", i32), ("NgatherGlobal", i64), ("setup_seconds", dbl),
-                ("scatter_by_rows", i32), ("mixed_tiles", i32 * 2)]
+                ("scatter_by_rows", i32), ("mixed_tiles", i32 * 2),
+                ("gs_nontemporal", i32)]
 
 
 class GsTuning(C.Structure):
@@ -70,7 +71,7 @@ class CgTuning(C.Structure):
 class Srb(C.Structure):
     """spx_srb_t: device addresses of a sorted-run tile layout."""
     _fields_ = [("nblocks", i32), ("chunks_per_tile", i32), ("xcd_remap", i32), ("tile_lanes", i32),
-                ("chunks", vp), ("runIds", vp), ("links", vp)]
+                ("nontemporal", i32), ("chunks", vp), ("runIds", vp), ("links", vp)]
 
 
 # name: (restype, argtypes) -- every function of include/spx.h
@@ -160,6 +161,7 @@ SPXH_FUNCS = {
     "spxh_ogs_set_gs_row_keys": (C.c_int, [vp, vp]),
     "spxh_ogs_set_gs_tiles": (C.c_int, [vp, vp]),
     "spxh_ogs_set_gs_mixed_tiles": (C.c_int, [vp, C.c_int, vp]),
+    "spxh_ogs_set_gs_nontemporal": (C.c_int, [vp, C.c_int]),
     "spxh_hex_gs_tiles": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32, vp, vp, vp]),
     "spxh_ogs_setup_batch": (C.c_int, [vp, i32, C.POINTER(vp)]),
     "spxh_ogs_setup_hex": (C.c_int, [i32, i32, i32, C.c_int, C.c_int, C.c_int, vp, C.POINTER(vp)]),
@@ -175,6 +177,7 @@ SPXH_FUNCS = {
     "spxh_srb_destroy": (C.c_int, [vp]),
     "spxh_srb_build_signed": (C.c_int, [i32, vp, vp, i32, vp, i32, i32, i32, vp, vp, C.POINTER(vp)]),
     "spxh_srb_copy_members": (C.c_int, [vp, vp]),
+    "spxh_srb_nontemporal": (C.c_int, [vp, C.POINTER(i32)]),
     "spxh_srb_rows_ascending": (C.c_int, [vp, vp]),
     "spxh_ogs_gather": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]),
     "spxh_ogs_scatter": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp]),

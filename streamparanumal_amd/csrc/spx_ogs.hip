@@ -168,6 +168,9 @@ int srb_launch_setup(const char *name, int type, int K, const spx_srb_t *srb, La
   if (srb->xcd_remap >= 0 && !xcd_hint(srb->xcd_remap, L.nxcd > 1, L.nxcd, L.xcdLog2))
     return fail(SPX_ERR_ARG, std::string(name) + ": xcd_remap must be negative (library default), 0, 1 or a power of two up to 1024");
   if (srb->tile_lanes) L.variant = srb->tile_lanes == 64 ? 2 : 0;
+  if (srb->nontemporal & ~0x3e)
+    return fail(SPX_ERR_ARG, std::string(name) + ": nontemporal takes bits 1..5 (nodes of 2..32 bytes)");
+  L.ntMask = srb->nontemporal;
   A.chunks = reinterpret_cast<const SrbChunk *>(srb->chunks);
   A.runIds = srb->runIds;
   A.links = srb->links;

@@ -88,6 +88,16 @@ static int srb_shape(const Launch &L, int op, F &&f, bool gs = false) {
   });
 }
 
+// Non-temporal values (L.ntMask, spx_srb_t.nontemporal) for a Sym pass over
+// all K fields of nodes of K*sizeof(T) = 2^e bytes with bit e set.
+static bool srb_nontemporal(const Launch &W) {
+  const int bytes = W.K * (int)sizeof(T);
+  if (W.kf != W.K || bytes > 32 || (bytes & (bytes - 1))) return false;
+  int e = 0;
+  while ((1 << e) < bytes) ++e;
+  return (W.ntMask >> e) & 1;
+}
+
 // mode (kAll, kFoldN, kStoreN) and memberN: the N-CSR forms, on 256-lane
 // tiles only (one instance set per mode).
 int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q, int mode, const unsigned long long *memberN) {
@@ -96,7 +106,10 @@ int launch_gs_srb(const Launch &L, int op, const SrbArgs &A, T *q, int mode, con
   return srb_shape<true>(
       L2, op,
       [&]<int OPC, int KF, int PER>(const Launch &W, bool wave, size_t lds) {
-        if (mode == kFoldN)
+        if (mode == kAll && !wave && srb_nontemporal(W))
+          hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, kThreads, Fold<T, OPC>, kAll, true>), grid_of(W),
+                             dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), q, nullptr);
+        else if (mode == kFoldN)
           hipLaunchKernelGGL((k_gs_srb<T, OPC, KF, PER, kThreads, Fold<T, OPC>, kFoldN>), grid_of(W),
                              dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), q, memberN);
         else if (mode == kStoreN)

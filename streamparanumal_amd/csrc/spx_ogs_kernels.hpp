@@ -51,6 +51,7 @@ struct Launch {
   hipStream_t stream;
   int *err;
   int rows = 0;  // sorted-run N-CSR gather / scatter: rows of the N-CSR
+  int ntMask = 0;  // sorted-run gather+scatter: spx_srb_t.nontemporal
 };
 
 template <typename T> struct Lim;
@@ -86,6 +87,30 @@ template <typename T> struct Fold<T, SPX_MIN> {
   static __device__ __forceinline__ void apply(T &a, T b) { a = (b < a) ? b : a; }
   static __device__ __forceinline__ T out(T a) { return a; }
 };
+
+// Non-temporal (streaming) load and store of one value; 2-byte wrapper types
+// go through their bits.
+template <typename T> __device__ __forceinline__ T nt_load(const T *p) {
+  if constexpr (std::is_arithmetic<T>::value) {
+    return __builtin_nontemporal_load(p);
+  } else {
+    static_assert(sizeof(T) == 2, "2-byte wrapper");
+    const unsigned short b = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+    T r;
+    __builtin_memcpy(&r, &b, 2);
+    return r;
+  }
+}
+template <typename T> __device__ __forceinline__ void nt_store(T x, T *p) {
+  if constexpr (std::is_arithmetic<T>::value) {
+    __builtin_nontemporal_store(x, p);
+  } else {
+    static_assert(sizeof(T) == 2, "2-byte wrapper");
+    unsigned short b;
+    __builtin_memcpy(&b, &x, 2);
+    __builtin_nontemporal_store(b, reinterpret_cast<unsigned short *>(p));
+  }
+}
 
 __device__ __forceinline__ void flag_error(int *err, int code) {
   if (threadIdx.x == 0) err[0] = code;
@@ -397,7 +422,8 @@ template <typename T> struct GsLds<T, 1, true> {
   __device__ int get_head(int i) const { return (int)((unsigned)e[i].next >> 16); }
 };
 
-template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>, int MODE = kAll>
+template <typename T, int OP, int KF, int PER, int NT = kThreads, typename F = Fold<T, OP>, int MODE = kAll,
+          bool NTV = false>
 __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restrict__ q,
                                                const unsigned long long *__restrict__ memberN) {
   // Several 4- or 8-byte fields per pass: each lane loads and stores its
@@ -414,6 +440,12 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
   // of KF separate 2-byte accesses per lane (BF16, K = 3: 13 % faster), and
   // the LDS slot of word w is the chunk's base slot + w.
   //
+  // NTV: the values are loaded and stored non-temporally.  Each value is
+  // read and written once per call, so where the tiles' runs cover whole
+  // 128-byte lines, streaming them keeps the caches (L2 and the Infinity
+  // Cache) for the layout tables, which every call reads again; where lines
+  // are shared between tiles it would cost their reuse instead.
+  //
   // MODE kFoldN / kStoreN (memberN given): positions outside the N-CSR hold
   // the op's identity in the fold (an exact no-op: the sum starts from +0 and
   // so is never -0, x*1 == x, and the max / min never pass -/+ the type's
@@ -421,6 +453,7 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
   constexpr bool kWord = KF > 1 && sizeof(T) == 2;
   static_assert(MODE == kAll || (!kWord && std::is_same<typename F::Acc, T>::value),
                 "N-CSR forms fold plain 4- and 8-byte values");
+  static_assert(!NTV || (MODE == kAll && !kWord), "non-temporal values: Sym form, value-major fields");
   constexpr int LF = kWord ? KF : 1;  // fields held in LDS per position
   // several 4-byte fields: heads kept beside the links (one walk per field)
   constexpr bool kHeadLink = KF > 1 && srb_gs_packed<T>(KF) && sizeof(T) == 4;
@@ -461,6 +494,7 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
     for (int f = 0; f < KF; ++f) {
       if constexpr (!kWord) {
         if constexpr (MODE == kFoldN) v[j][f] = ((ld >> j) & 1u) ? q[(int64_t)id[j] * K + kb + f] : F::init();
+        else if constexpr (NTV) v[j][f] = nt_load(&q[(int64_t)id[j] * K + kb + f]);
         else v[j][f] = q[(int64_t)id[j] * K + kb + f];
       } else {
         const int w = f * 64 + l, src = w / KF;
@@ -559,12 +593,17 @@ __global__ __launch_bounds__(NT) void k_gs_srb(Launch L, SRB_PARAMS, T *__restri
       // malformed layout, keeps its own)
       if ((st >> j) & 1u) {
         const int hd = S.get_next(t + j * NT);
-        q[(int64_t)id[j] * K + kb] = S.get_val((unsigned)hd < (unsigned)cap ? hd : t + j * NT, 0);
+        const T r = S.get_val((unsigned)hd < (unsigned)cap ? hd : t + j * NT, 0);
+        if constexpr (NTV) nt_store(r, &q[(int64_t)id[j] * K + kb]);
+        else q[(int64_t)id[j] * K + kb] = r;
       }
     } else if constexpr (!kWord) {
       if ((st >> j) & 1u)
 #pragma unroll
-        for (int f = 0; f < KF; ++f) q[(int64_t)id[j] * K + kb + f] = v[j][f];
+        for (int f = 0; f < KF; ++f) {
+          if constexpr (NTV) nt_store(v[j][f], &q[(int64_t)id[j] * K + kb + f]);
+          else q[(int64_t)id[j] * K + kb + f] = v[j][f];
+        }
     } else {
       const unsigned long long vm = __ballot((valid >> j) & 1u);
 #pragma unroll

@@ -39,6 +39,12 @@ struct SortedRunBlocks {
   bool rowsAscending = true;
   int xcdRemap = -1;  // launch hints passed in spx_srb_t (-1 / 0: library default)
   int tileLanes = 0;
+  // non-temporal value access of the Sym gather+scatter (spx_srb_t.nontemporal):
+  // ntMask bit e set by the builder when at least kSrbWholeLineShare of the
+  // tiles' accesses to 128-byte lines of 2^e-byte nodes take whole lines;
+  // ntHint -1 uses ntMask, 0 none, 1 every node size
+  int ntMask = 0;
+  int ntHint = -1;
   memory<dlong> entryStarts;  // host only: CSR entries of each tile
   memory<SrbChunkRecord> chunks;
   memory<dlong> runIds;
@@ -58,6 +64,13 @@ struct SortedRunBlocks {
 
 // chunk records per tile for a row block capacity (a power of two >= 4)
 int SrbChunksPerTile(int cap);
+// share of whole lines (tile, 128-byte line) among the tiles' line accesses
+// from which a node size gets non-temporal values.  Measured on MI355X, FP64
+// hex boxes at ~1e8 nodes, us per call with ordinary -> non-temporal values
+// (share of 8-byte nodes): p=7 line tiles 310 -> 265 (1.00; 64^3: 407 ->
+// 346), p=15 300 -> 282 (0.87), p=11 336 -> 374 (0.69-0.77), p=9, 10
+// 349 / 343 -> 435 / 469 (0.57-0.59), p=1..6 1.2-1.8x slower (<= 0.36).
+constexpr double kSrbWholeLineShare = 0.85;
 
 // memberFlag (one byte per local node, optional): fills memberN.
 SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowStarts, const memory<dlong> colIds,
@@ -136,8 +149,9 @@ class ogsOperator_t {
   // The FP32 and BF16 gather+scatter (GatherScatterMixedRaw) may take a tile
   // order of their own: gsMixed[0] for SPX_STORE_F32, gsMixed[1] for
   // SPX_STORE_BF16 (own = false: the gsSrb tiles).  The launch hints are
-  // gsSrb's.  A hex box's setup gives them line tiles where those measured
-  // faster for the narrower values (HexGsLineRowsMixed).
+  // gsSrb's.  A hex box's setup gives them line tiles, or lattice blocks
+  // where FP64 takes line tiles, where those measured faster for the
+  // narrower values (HexGsLineRowsMixed).
   struct MixedTiles {
     bool own = false;
     memory<dlong> order;
@@ -260,8 +274,8 @@ void HexGsBlockShape(const int p, int &bx, int &by, int &bz);
 // tiles (HexLineTiles), or 0 for the lattice blocks of HexGsBlockShape.
 int HexGsLineRows(const int p);
 // Node rows per line tile of the FP32 (storage SPX_STORE_F32) or BF16
-// (SPX_STORE_BF16) gather+scatter of a hex box of degree p, or 0 for the
-// tiles of the FP64 one.
+// (SPX_STORE_BF16) gather+scatter of a hex box of degree p, 0 for the tiles
+// of the FP64 one, or -1 for the lattice blocks of HexGsBlockShape.
 int HexGsLineRowsMixed(const int p, const int storage);
 // Gather+scatter tiles of an nx x ny x nz hex box at degree p whose members
 // fill whole memory lines of the vector: a line holds rowsPerLine node rows

@@ -163,6 +163,7 @@ int spxh_ogs_info(spxh_ogs_t h, spxh_ogs_info_t *info) {
     info->scatter_by_rows = op->scatterByRows ? 1 : 0;
     info->mixed_tiles[0] = op->gsMixed[0].own ? 1 : 0;
     info->mixed_tiles[1] = op->gsMixed[1].own ? 1 : 0;
+    info->gs_nontemporal = op->gsSrb.Device().nontemporal;
   });
 }
 
@@ -222,6 +223,16 @@ int spxh_ogs_set_gs_format(spxh_ogs_t h, int sorted, int xcd_remap) {
     op->gsSorted = sorted != 0;
     op->gsSrb.xcdRemap = xcd_remap;
     op->rowSrb.xcdRemap = xcd_remap;
+  });
+}
+
+int spxh_ogs_set_gs_nontemporal(spxh_ogs_t h, int mode) {
+  if (!h) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_nontemporal: null");
+  if (mode < -1 || mode > 1) return err(SPX_ERR_ARG, "spxh_ogs_set_gs_nontemporal: mode must be -1, 0 or 1");
+  return guarded([&] {
+    auto op = h->ogs.localOperator();
+    LIBP_ABORT("ogs not set up", !op);
+    op->gsSrb.ntHint = mode;
   });
 }
 
@@ -527,6 +538,12 @@ int spxh_srb_copy_members(spxh_srb_t h, uint64_t *memberN) {
   const ogs::SortedRunBlocks &s = h->srb;
   if (!s.memberN.length()) return err(SPX_ERR_ARG, "spxh_srb_copy_members: layout built without member flags");
   if (s.nchunks) std::copy(s.memberN.ptr(), s.memberN.ptr() + s.nchunks, memberN);
+  return SPX_OK;
+}
+
+int spxh_srb_nontemporal(spxh_srb_t h, int32_t *mask) {
+  if (!h || !mask) return err(SPX_ERR_ARG, "spxh_srb_nontemporal: null");
+  *mask = h->srb.ntMask;
   return SPX_OK;
 }
 

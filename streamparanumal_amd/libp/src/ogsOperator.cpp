@@ -126,9 +126,11 @@ SortedRunBlocks ogsOperator_t::BuildOrderedSrb(const memory<dlong> &order, const
 }
 
 void ogsOperator_t::BuildGsSrb(int xcd, int lanes) {
+  const int nt = gsSrb.ntHint;
   gsSrb = BuildOrderedSrb(gsRowOrder, gsTileCut);
   gsSrb.xcdRemap = xcd;
   gsSrb.tileLanes = lanes;
+  gsSrb.ntHint = nt;
 }
 
 void ogsOperator_t::SetGsMixedRowOrder(const int storage, const memory<dlong> &order, const memory<uint8_t> &cuts,
@@ -294,14 +296,16 @@ memory<dlong> CountingOrder(const dlong Nrows, const std::vector<int32_t> &key, 
 }
 
 int HexGsLineRows(const int p) {
-  // Measured on MI355X, us per call, setup's lattice blocks -> line tiles of
-  // 2 node rows (128-byte lines of FP64 values), FP64 / FP32 / BF16 storage:
-  // 64^3 p=7 481 / 351 / 364 -> 460 / 331 / 348 (three runs: 4-6 % for
-  // FP64), 58^3 p=7 365 -> 356; at other degrees the blocks were as fast or
-  // faster for FP64 (~1e8 nodes: p=1 483 -> 653, p=8 386 -> 443, p=9 370 ->
-  // 396, p=15 338 -> 344, p=3 409 -> 404; 1 or 4 rows per tile were no
-  // better), so only p=7 takes them.
-  return p == 7 ? 2 : 0;
+  // Line tiles cover whole 128-byte lines of FP64 values where an element's
+  // node rows fill them: 2 rows at p=7, 1 row at p=15.  Measured on MI355X,
+  // us per call, setup's lattice blocks -> line tiles, FP64 / FP32 / BF16
+  // storage: 64^3 p=7 481 / 351 / 364 -> 460 / 331 / 348 (three runs: 4-6 %
+  // for FP64), 58^3 p=7 365 -> 356; with the non-temporal values whole
+  // lines allow (kSrbWholeLineShare): 29^3 p=15 FP64 283 (blocks) -> 259
+  // (1 row; 261 with 2), 64^3 p=7 346.  At other degrees the blocks were as
+  // fast or faster for FP64 (~1e8 nodes: p=1 483 -> 653, p=8 386 -> 443,
+  // p=9 370 -> 396, p=3 409 -> 404; 1 or 4 rows per tile were no better).
+  return p == 7 ? 2 : p == 15 ? 1 : 0;
 }
 
 int HexGsLineRowsMixed(const int p, const int storage) {
@@ -309,7 +313,10 @@ int HexGsLineRowsMixed(const int p, const int storage) {
   // tiles: BF16 p=1 376 -> 253 (1 row), p=2 382 -> 342 (2 rows; 428 with 1),
   // p=7 267 -> 254 (1 row; 64^3: 346 -> 329); FP32 p=7 247 -> 241 (64^3: 318
   // -> 307).  FP32 lost at p=1 (363 -> 401) and p=2; at the other degrees
-  // either storage was within 2.5 % or slower.
+  // either storage was within 2.5 % or slower.  At p=15 both keep the
+  // lattice blocks (-1) where FP64 takes line tiles: 29^3 FP32 / BF16 191 /
+  // 186 against 197 / 191 on the FP64 line tiles.
+  if (p == 15) return -1;
   if (storage == SPX_STORE_BF16) return p == 1 ? 1 : p == 2 ? 2 : p == 7 ? 1 : 0;
   if (storage == SPX_STORE_F32) return p == 7 ? 1 : 0;
   return 0;

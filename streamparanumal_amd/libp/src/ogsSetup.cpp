@@ -298,16 +298,24 @@ void ogsBase_t::SetupHexBox(const dlong nx, const dlong ny, const dlong nz, cons
       HexGsBlockShape(p, bx, by, bz);
       gatherLocal->SetGsRowOrder(LatticeBlockOrder(op.NrowsT, grid, Mx, My, Mz, bx, by, bz), false);
     }
-    // FP32 / BF16 tiles of their own where line tiles of other heights were
-    // faster for them (one layout where both storages take the same)
+    // FP32 / BF16 tiles of their own where line tiles of other heights, or
+    // the lattice blocks (r < 0), were faster for them than the FP64 tiles
+    // (one layout where both storages take the same)
     memory<dlong> mixedOrder;
     memory<uint8_t> mixedCuts;
     int mixedRows = 0;
     for (int s : {SPX_STORE_F32, SPX_STORE_BF16}) {
       const int r = HexGsLineRowsMixed(p, s);
-      if (r <= 0 || r == rowsPerLine) continue;
+      if (r == 0 || r == rowsPerLine || (r < 0 && rowsPerLine <= 0)) continue;
       if (r != mixedRows) {
-        HexLineTiles(op.NrowsT, grid, nx, ny, nz, p, r, (int)op.gsBlockNnz, mixedOrder, mixedCuts);
+        if (r > 0) {
+          HexLineTiles(op.NrowsT, grid, nx, ny, nz, p, r, (int)op.gsBlockNnz, mixedOrder, mixedCuts);
+        } else {
+          int bx, by, bz;
+          HexGsBlockShape(p, bx, by, bz);
+          mixedOrder = LatticeBlockOrder(op.NrowsT, grid, Mx, My, Mz, bx, by, bz);
+          mixedCuts = memory<uint8_t>();
+        }
         mixedRows = r;
       }
       gatherLocal->SetGsMixedRowOrder(s, mixedOrder, mixedCuts, false);

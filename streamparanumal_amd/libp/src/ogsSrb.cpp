@@ -41,12 +41,16 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
   memory<dlong> prevPos(S.nnz > 0 ? S.nnz : 1);  // previous member's position, or -1
   std::vector<dlong> nruns(Nblocks, 0), nlinks(Nblocks, 0);
   int duplicate = 0, unordered = 0;
+  // per node size 2^e bytes: (tile, 128-byte line) pairs with an entry, and
+  // those holding the whole line
+  int64_t lineAcc[6] = {0, 0, 0, 0, 0, 0}, wholeAcc[6] = {0, 0, 0, 0, 0, 0};
 
   // pass 1: sort each tile by local index, link row members, masks and counts
 #pragma omp parallel
   {
     std::vector<std::pair<dlong, dlong>> ent;
     std::vector<dlong> posOfSlot;
+    int64_t lines[6] = {0, 0, 0, 0, 0, 0}, whole[6] = {0, 0, 0, 0, 0, 0};
 #pragma omp for schedule(dynamic, 64)
     for (dlong b = 0; b < Nblocks; ++b) {
       const dlong e0 = S.entryStarts[b], n = S.entryStarts[b + 1] - e0;
@@ -54,6 +58,17 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
       posOfSlot.resize(n);
       for (dlong i = 0; i < n; ++i) ent[i] = {colIds[e0 + i], i};
       std::sort(ent.begin(), ent.end());
+      for (int e = 1; e <= 5; ++e) {
+        const int sh = 7 - e;  // 2^sh nodes per line
+        for (dlong i = 0; i < n;) {
+          const int64_t line = (int64_t)ent[i].first >> sh;
+          dlong j = i + 1;
+          while (j < n && ((int64_t)ent[j].first >> sh) == line) ++j;
+          ++lines[e];
+          whole[e] += (j - i == ((dlong)1 << sh));
+          i = j;
+        }
+      }
       for (dlong i = 0; i < n; ++i) {
         sorted[e0 + i] = ent[i].first;
         posOfSlot[ent[i].second] = i;
@@ -105,9 +120,18 @@ SortedRunBlocks BuildSortedRunBlocks(const dlong Nrows, const memory<dlong> rowS
       nruns[b] = runs;
       nlinks[b] = links;
     }
+    for (int e = 1; e <= 5; ++e) {
+#pragma omp atomic
+      lineAcc[e] += lines[e];
+#pragma omp atomic
+      wholeAcc[e] += whole[e];
+    }
   }
   LIBP_ABORT("a local node appears twice in one row block", duplicate);
   S.rowsAscending = !unordered;
+  S.ntMask = 0;
+  for (int e = 1; e <= 5; ++e)
+    if (lineAcc[e] > 0 && (double)wholeAcc[e] >= kSrbWholeLineShare * (double)lineAcc[e]) S.ntMask |= 1 << e;
   // pass 2: run and link tables
   std::vector<dlong> runStart(Nblocks + 1, 0), linkStart(Nblocks + 1, 0);
   for (dlong b = 0; b < Nblocks; ++b) {
@@ -188,6 +212,7 @@ spx_srb_t SortedRunBlocks::Device() const {
   d.chunks_per_tile = cpt;
   d.xcd_remap = xcdRemap;
   d.tile_lanes = tileLanes;
+  d.nontemporal = ntHint < 0 ? ntMask : ntHint ? 0x3e : 0;
   d.chunks = reinterpret_cast<const spx_srb_chunk_t *>(o_chunks.ptr());
   d.runIds = o_runIds.ptr();
   d.links = o_links.ptr();

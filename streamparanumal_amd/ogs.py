@@ -139,6 +139,9 @@ def srb_build(row_starts, col_ids, block_starts, cap, n_csr=None, ncols=None):
         asc = C.c_int32()
         checkh(libspxhost().spxh_srb_rows_ascending(h, C.byref(asc)), "spxh_srb_rows_ascending")
         out["rows_ascending"] = bool(asc.value)  # the gather and scatter need it
+        nt = C.c_int32()
+        checkh(libspxhost().spxh_srb_nontemporal(h, C.byref(nt)), "spxh_srb_nontemporal")
+        out["nontemporal"] = nt.value  # spx_srb_t.nontemporal as the builder sets it
         if n_csr is not None:
             out["memberN"] = np.zeros(max(nch, 1), np.uint64)
             checkh(libspxhost().spxh_srb_copy_members(h, out["memberN"].ctypes.data), "spxh_srb_copy_members")
@@ -151,7 +154,7 @@ def srb_build(row_starts, col_ids, block_starts, cap, n_csr=None, ncols=None):
 class SrbDevice:
     """A sorted-run layout uploaded to the device; `.desc` is the spx_srb_t to pass by reference."""
 
-    def __init__(self, srb, xcd_remap=-1, tile_lanes=0):
+    def __init__(self, srb, xcd_remap=-1, tile_lanes=0, nontemporal=None):
         from .device import DeviceArray
         self.arrays = {"chunks": DeviceArray.from_numpy(srb["chunks"].view(np.uint8)),
                        "runIds": DeviceArray.from_numpy(srb["runIds"]),
@@ -159,7 +162,8 @@ class SrbDevice:
         if "memberN" in srb:
             self.arrays["memberN"] = DeviceArray.from_numpy(srb["memberN"])
             self.nrowsN = srb["nrowsN"]
-        self.desc = _lib.Srb(srb["nblocks"], srb["chunks_per_tile"], xcd_remap, tile_lanes,
+        nt = srb.get("nontemporal", 0) if nontemporal is None else nontemporal
+        self.desc = _lib.Srb(srb["nblocks"], srb["chunks_per_tile"], xcd_remap, tile_lanes, nt,
                              *(self.arrays[k].ptr for k in ("chunks", "runIds", "links")))
 
     def gather_scatter_signed(self, q, k=1, op=_lib.ADD, trans=_lib.NOTRANS):
@@ -290,16 +294,24 @@ class Ogs:
         checkh(libspxhost().spxh_ogs_set_gs_format(self.h, 1 if sorted_runs else 0, xcd_remap),
                "spxh_ogs_set_gs_format")
 
+    def set_gs_nontemporal(self, mode=-1):
+        """Non-temporal value loads and stores of the Sym gather+scatter: -1 where the tiles' builder
+        set them (node sizes whose 128-byte lines lie whole in one tile), 0 never, 1 always.  Results
+        do not change."""
+        checkh(libspxhost().spxh_ogs_set_gs_nontemporal(self.h, mode), "spxh_ogs_set_gs_nontemporal")
+        self.refresh()
+
     def set_gs_row_keys(self, keys=None):
         """Rows of the sorted-run gather+scatter tiles sorted by keys (one int64 per T-CSR row, ties
         in row order); None restores the CSR's order.  Results do not change."""
         if keys is None:
             checkh(libspxhost().spxh_ogs_set_gs_row_keys(self.h, None), "spxh_ogs_set_gs_row_keys")
-            return
-        k = np.ascontiguousarray(keys, dtype=np.int64)
-        if k.size != self.NlocalT:
-            raise ValueError(f"need {self.NlocalT} keys, got {k.size}")
-        checkh(libspxhost().spxh_ogs_set_gs_row_keys(self.h, k.ctypes.data), "spxh_ogs_set_gs_row_keys")
+        else:
+            k = np.ascontiguousarray(keys, dtype=np.int64)
+            if k.size != self.NlocalT:
+                raise ValueError(f"need {self.NlocalT} keys, got {k.size}")
+            checkh(libspxhost().spxh_ogs_set_gs_row_keys(self.h, k.ctypes.data), "spxh_ogs_set_gs_row_keys")
+        self.refresh()
 
     def set_gs_tiles(self, keys):
         """Rows of the sorted-run gather+scatter tiles sorted by keys, and no tile holding rows of two
@@ -308,6 +320,7 @@ class Ogs:
         if k.size != self.NlocalT:
             raise ValueError(f"need {self.NlocalT} keys, got {k.size}")
         checkh(libspxhost().spxh_ogs_set_gs_tiles(self.h, k.ctypes.data), "spxh_ogs_set_gs_tiles")
+        self.refresh()
 
     def set_gs_mixed_tiles(self, storage, keys=None):
         """As set_gs_tiles for the FP32 (storage STORE_F32) or BF16 (STORE_BF16) gather+scatter alone;

@@ -99,8 +99,10 @@ def test_gs_mixed_f64_is_plain(setups, oracle):
     assert np.array_equal(dq.to_numpy().view(np.uint64), o.gather_scatter(q).view(np.uint64))
 
 
-# Hex boxes set up in closed form: FP32 / BF16 tiles of their own at p = 1, 2 (BF16) and 7 (both)
-HEX_OWN = {(6, 5, 4, 1): [0, 1], (5, 4, 3, 2): [0, 1], (12, 6, 3, 7): [1, 1], (4, 3, 3, 3): [0, 0]}
+# Hex boxes set up in closed form: FP32 / BF16 tiles of their own at p = 1, 2 (BF16), 7 (both, line
+# tiles) and 15 (both, lattice blocks under FP64's line tiles)
+HEX_OWN = {(6, 5, 4, 1): [0, 1], (5, 4, 3, 2): [0, 1], (12, 6, 3, 7): [1, 1], (4, 3, 3, 3): [0, 0],
+           (3, 2, 2, 15): [1, 1]}
 
 
 @pytest.fixture(scope="module")

@@ -136,6 +136,29 @@ def test_all_types_ops_trans_fields(setups, dtype, op, trans, k):
     assert same(dv.to_numpy(), o.scatter(gsrc, base, k, trans))
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_gs_nontemporal_values(setups, dtype, op, k):
+    """Sym gather+scatter with non-temporal value loads and stores forced on for every node size (one
+    pass over K = 1, 2, 4 fields; K = 3 keeps ordinary ones): the oracle's bits."""
+    mesh = (3, 2, 1, 3)
+    say("gs nontemporal", mesh, np.dtype(dtype).name, op, k)
+    g, o, _ = setups[mesh]
+    q = op_values(g.N, dtype, 8, k, op)
+    g.set_gs_nontemporal(1)
+    try:
+        assert g.info["gs_nontemporal"] == 0x3E
+        dq = DeviceArray.from_numpy(q)
+        g.gather_scatter(dq, k, op, spx.SYM)
+        device.check_kernel_errors()
+        assert same(dq.to_numpy(), o.gather_scatter(q, k, op, spx.SYM))
+    finally:
+        g.set_gs_nontemporal(-1)
+    with pytest.raises(SpxError):
+        g.set_gs_nontemporal(2)
+
+
 @pytest.mark.parametrize("cap", [64, 512, 1024, 3000, 4096])
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, "srb"])
 def test_result_independent_of_row_blocks(setups, oracle, cap, variant):
@@ -264,6 +287,14 @@ def test_srb_direct_random_ids(dev, oracle, seed, cap):
     for bad in (3, 2048):
         with pytest.raises(SpxError):
             ogs.SrbDevice(srb, xcd_remap=bad).gather_scatter(DeviceArray.from_numpy(q))
+    for nt in (0, 0x08, 0x3E):                      # non-temporal values: none, 8-byte nodes, every size
+        dq = DeviceArray.from_numpy(q)
+        ogs.SrbDevice(srb, nontemporal=nt).gather_scatter(dq)
+        device.check_kernel_errors()
+        assert same(dq.to_numpy(), want)
+    for bad in (1, 0x40):
+        with pytest.raises(SpxError):
+            ogs.SrbDevice(srb, nontemporal=bad).gather_scatter(DeviceArray.from_numpy(q))
     # gather then scatter on the layout reproduces the gather+scatter
     nrows = rs.size - 1
     dg = DeviceArray(nrows)

@@ -304,7 +304,8 @@ def test_hex_box_setup_with_a_smaller_thread_team():
 
 @pytest.mark.parametrize("box,r,cap", [((6, 5, 4, 7), 1, 2048), ((6, 5, 4, 7), 2, 2048), ((6, 5, 4, 7), 4, 4096),
                                        ((9, 3, 2, 3), 2, 2048), ((4, 4, 4, 8), 2, 2048), ((20, 3, 3, 7), 2, 2048),
-                                       ((64, 3, 2, 7), 2, 2048)])
+                                       ((64, 3, 2, 7), 2, 2048), ((6, 3, 2, 15), 1, 2048),
+                                       ((30, 2, 2, 15), 1, 2048)])
 def test_hex_line_tiles(box, r, cap):
     """Line-aligned gather+scatter tiles: a permutation of the rows cut into tiles of at most cap
     members, in which every group of r node rows of an element (one memory line of the vector) lies
@@ -335,3 +336,69 @@ def test_hex_line_tiles(box, r, cap):
         assert split == 0
     else:
         assert 0 < split <= (nx * ny * nz * Nq * Nq // r) // 16   # only the element columns at x cuts
+
+
+def whole_line_mask(ci, entry_starts):
+    """spx_srb_t.nontemporal from its definition: bit e when at least 85 % of the (tile, 128-byte line)
+    pairs of a vector of 2^e-byte nodes that hold an entry hold all 2^(7-e) nodes of the line."""
+    tile = np.repeat(np.arange(entry_starts.size - 1), np.diff(entry_starts))
+    ids = ci[: entry_starts[-1]].astype(np.int64)
+    mask = 0
+    for e in range(1, 6):
+        sh = 7 - e
+        line = ids >> sh
+        _, counts = np.unique(tile * (int(line.max()) + 1) + line, return_counts=True)
+        if np.count_nonzero(counts == 1 << sh) >= 0.85 * counts.size:
+            mask |= 1 << e
+    return mask
+
+
+def tiles_from_order(rs, ci, order, cuts, cap):
+    """The CSR with its rows in `order`, and greedy row blocks of at most cap entries that also start at
+    every cut (blockRowsGreedy with cuts)."""
+    sizes = np.diff(rs)[order]
+    rs2 = np.zeros(order.size + 1, dtype=np.int64)
+    rs2[1:] = np.cumsum(sizes)
+    ci2 = np.concatenate([ci[rs[r]:rs[r + 1]] for r in order])
+    bs, acc = [0], 0
+    for i, sz in enumerate(sizes):
+        if i > 0 and (cuts[i] or acc + sz > cap):
+            bs.append(i)
+            acc = 0
+        acc += sz
+    bs.append(order.size)
+    return rs2, ci2, np.asarray(bs)
+
+
+@pytest.mark.parametrize("case", ["line2", "line1", "blocks_p1", "random"])
+def test_srb_nontemporal_mask(case):
+    """The layout builder's non-temporal mask equals its definition on hex line tiles of 2 node rows at
+    p = 7 (whole 128-byte lines of FP64 nodes: bit 3), 1-row line tiles, lattice blocks at p = 1 and a
+    CSR of random ids (short runs: no bit)."""
+    cap = 2048
+    if case == "random":
+        ids = (o.rand_int(30000, 7) % 9000 + 1).astype(np.int64)
+        _, c = ogs.build_csr(ids)
+        rs, ci = c["rowStartsT"], c["colIdsT"]
+        bs = ogs.block_rows(rs, cap)
+    else:
+        nx, ny, nz, p = (64, 3, 2, 7) if case.startswith("line") else (24, 8, 8, 1)
+        ids, c = ogs.build_csr_hex(nx, ny, nz, p, spx.SIGNED, True)
+        rs, ci = c["rowStartsT"], c["colIdsT"]
+        grid = np.abs(ids[ci[rs[:-1]]]) - 1
+        if case.startswith("line"):
+            order, cuts = ogs.hex_gs_tiles(nx, ny, nz, p, 2 if case == "line2" else 1, cap, grid)
+        else:
+            Mx, My = nx * p + 1, ny * p + 1                  # 8 x 4 x 4 blocks of lattice points
+            i, j, k = grid % Mx, (grid // Mx) % My, grid // (Mx * My)
+            key = ((k // 4) * (-(-My // 4)) + j // 4) * (-(-Mx // 8)) + i // 8
+            order = np.argsort(key, kind="stable").astype(np.int32)
+            cuts = np.zeros(order.size, np.uint8)
+        rs, ci, bs = tiles_from_order(rs, ci, order, cuts, cap)
+    srb = ogs.srb_build(rs, ci, bs, cap)
+    want = whole_line_mask(ci, srb["entryStarts"])
+    assert srb["nontemporal"] == want
+    if case == "line2":
+        assert want & 0x08                        # FP64 nodes: whole lines
+    if case in ("blocks_p1", "random"):
+        assert want & 0x0E == 0                   # 2-, 4- and 8-byte nodes: split lines
