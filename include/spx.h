@@ -253,7 +253,8 @@ typedef struct {
                               tiles, 0: hardware order, negative: library default */
   int32_t tile_lanes;      /* lanes per tile: 64 (one wave, <= 1024 positions) or 256; 0: library default */
   int32_t nontemporal;     /* bit e (e = 1..5): the Sym gather+scatter (spx_ogs_gather_scatter_srb,
-                              spx_ogs_gather_scatter_srb_signed with SPX_SYM) streams the values of
+                              spx_ogs_gather_scatter_srb_signed with SPX_SYM,
+                              spx_ogs_gather_scatter_srb_mixed) streams the values of
                               nodes of 2^e bytes (K values of the type, all K in one pass) past the
                               caches with non-temporal loads and stores; the layout builders set
                               the bits for which whole 128-byte lines make up most of the tiles'

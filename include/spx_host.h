@@ -112,8 +112,8 @@ int spxh_ogs_set_gs_tiles(spxh_ogs_t ogs, const int64_t *keys);
    gives them line tiles of their own where those measured faster.  No
    reference counterpart; results are identical. */
 int spxh_ogs_set_gs_mixed_tiles(spxh_ogs_t ogs, int storage, const int64_t *keys);
-/* Non-temporal value loads and stores of the Sym gather+scatter
-   (spx_srb_t.nontemporal): -1 as the tiles' builder set them (node sizes
+/* Non-temporal value loads and stores of the Sym gather+scatter, FP64 and
+   FP32 / BF16 storage (spx_srb_t.nontemporal): -1 as the tiles' builder set them (node sizes
    whose 128-byte lines mostly lie whole in one tile), 0 never, 1 for every
    node size; kept when the tiles are rebuilt.  No reference counterpart;
    results are identical. */

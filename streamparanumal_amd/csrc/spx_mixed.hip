@@ -127,8 +127,22 @@ int launch_gs_srb_mixed(const Launch &L, const SrbArgs &A, S *q) {
   }
   const size_t lds = lds_of(W.kf);
   const dim3 grid(W.nblocks, W.K / W.kf);
+  // non-temporal values (spx_srb_t.nontemporal) for one pass over all K
+  // fields of K*sizeof(S) = 2^e bytes, value-major fields only
+  const int bytes = W.K * (int)sizeof(S);
+  int e = 0;
+  while ((1 << e) < bytes) ++e;
+  const bool ntv = !wave && W.kf == W.K && bytes <= 32 && (1 << e) == bytes && ((W.ntMask >> e) & 1) &&
+                   !(W.kf > 1 && sizeof(S) == 2);
 #define SPX_GSRM(KF, PER)                                                                                   \
   do {                                                                                                     \
+    if constexpr (KF == 1 || sizeof(S) == 4) {                                                             \
+      if (ntv) {                                                                                           \
+        hipLaunchKernelGGL((k_gs_srb<S, SPX_ADD, KF, PER, kThreads, FoldWide<S>, kAll, true>), grid,         \
+                           dim3(kThreads), lds, W.stream, W, SRB_ARGS(A), q, nullptr);                    \
+        break;                                                                                             \
+      }                                                                                                    \
+    }                                                                                                      \
     if (wave)                                                                                              \
       hipLaunchKernelGGL((k_gs_srb<S, SPX_ADD, KF, PER, 64, FoldWide<S>>), grid, dim3(64), lds, W.stream, W, \
                          SRB_ARGS(A), q, nullptr);                                                         \
@@ -410,6 +424,9 @@ int spx_ogs_gather_scatter_srb_mixed(int storage, int K, const spx_srb_t *srb, v
   L.kf = (K >= 2 && K <= 4) ? K : 1;
   L.per = 0;
   L.variant = srb->tile_lanes ? (srb->tile_lanes == 64 ? 2 : 0) : tu.variant;
+  if (srb->nontemporal & ~0x3e)
+    return fail(SPX_ERR_ARG, "spx_ogs_gather_scatter_srb_mixed: nontemporal takes bits 1..5 (nodes of 2..32 bytes)");
+  L.ntMask = srb->nontemporal;
   L.stream = current_stream();
   L.err = kernel_error_flag();
   SrbArgs A;

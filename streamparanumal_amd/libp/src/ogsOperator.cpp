@@ -453,6 +453,7 @@ void ogsOperator_t::GatherScatterMixedRaw(int storage, int k, void *v) {
     spx_srb_t d = own ? gsMixed[storage - 1].srb.Device() : gsSrb.Device();
     d.xcd_remap = gsSrb.xcdRemap;
     d.tile_lanes = gsSrb.tileLanes;
+    if (gsSrb.ntHint >= 0) d.nontemporal = gsSrb.ntHint ? 0x3e : 0;  // the operator's override
     SPX_CHECK(spx_ogs_gather_scatter_srb_mixed(storage, k, &d, v));
   } else if (NgsBlocks)
     SPX_CHECK(spx_ogs_gather_scatter_mixed(storage, k, NgsBlocks, o_gsBlockRowStarts.ptr(), o_rowStartsT.ptr(),

@@ -131,6 +131,22 @@ def test_gs_mixed_own_tiles(hex_setups, oracle, mesh, st, k):
     assert gs_mixed_matches(g, o, oracle, st, k)
 
 
+@pytest.mark.parametrize("mesh", [(12, 6, 3, 7), (3, 2, 2, 15), (4, 3, 3, 3)])
+@pytest.mark.parametrize("st", STORES)
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_gs_mixed_nontemporal(hex_setups, oracle, mesh, st, k):
+    """FP32 / BF16 gather+scatter with non-temporal values forced on (one pass over K = 1, and K = 2
+    for FP32; word-major BF16 fields and K = 3 keep ordinary ones) and off: the oracle's bits."""
+    say("gs mixed nontemporal", mesh, st, k)
+    g, o = hex_setups[mesh]
+    try:
+        for mode in (1, 0):
+            g.set_gs_nontemporal(mode)
+            assert gs_mixed_matches(g, o, oracle, st, k, seed=30 + mode)
+    finally:
+        g.set_gs_nontemporal(-1)
+
+
 @pytest.mark.parametrize("st", STORES)
 def test_gs_mixed_own_tiles_recut(dev, oracle, st):
     """Own tiles survive a capacity change, an FP64 and a mixed autotune, a reset and custom keys."""
