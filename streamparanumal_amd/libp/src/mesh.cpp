@@ -237,15 +237,17 @@ void mesh_t::BoxGatherScatterSetup(ogs::ogs_t &ogs, int type, dlong nx, dlong ny
   // ids, 1 + gx + gy*Gx [+ gz*Gx*Gy]) where that measured faster on MI355X at
   // ~1e8 nodes, us per call: Quad p=3 379 -> 347 and Tri p=3 394 -> 368 with
   // 12 x 12 (p=1, 2 likewise short-run); Tet p=1 513 -> 414 with 4 x 2 x 2,
-  // Tet p=3 626 -> 576 with 6 x 6 x 6.  At 2-D p=7 and Tet p=2 or 7 the CSR's
-  // order was as fast or faster.
+  // Tet p=3 626 -> 576 with 6 x 6 x 6, and later 514 -> 503 with 3 x 6 x 6
+  // (FP32 storage 363 -> 359; 2 x 6 x 6 502, 3 x 9 x 6 505, 6 x 3 x 6 540,
+  // 4 x 4 x 4 587).  At 2-D p=7 and Tet p=2 or 7 the CSR's order was as fast
+  // or faster.
   int bx = 0, by = 0, bz = 0;
   if ((type == TRIANGLES || type == QUADRILATERALS) && N <= 3) {
     bx = 12; by = 12; bz = 1;
   } else if (type == TETRAHEDRA && N == 1) {
     bx = 4; by = 2; bz = 2;
   } else if (type == TETRAHEDRA && N == 3) {
-    bx = 6; by = 6; bz = 6;
+    bx = 3; by = 6; bz = 6;
   }
   if (bx) {
     auto op = ogs.localOperator();

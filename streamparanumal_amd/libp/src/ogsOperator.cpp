@@ -329,14 +329,24 @@ void HexGsBlockShape(const int p, int &bx, int &by, int &bz) {
   // FP32 / BF16 storage (medians of 3 interleaved rounds): 469 / 347 / 370
   // -> 453 / 341 / 368 with 64x8x1, where 64x4x1 gives 460 / 335 / 357 and
   // 16x8x2 460 / 366 / 393.  From p=5 up the shapes tried (8x8x4 to
-  // 128x4x1) differ by at most 3 %, so every p >= 6 takes 64x8x1 (p=15
-  // 357 -> 304).
+  // 128x4x1) differ by at most 3 %, so p >= 7 takes 64x8x1 (p=15 357 ->
+  // 304; p = 7 and 15 take line tiles, HexGsLineRows).  Narrow blocks later did better at p = 1 and 2, whose tiles
+  // then take the x points of several blocks (~1e8 nodes, FP64 / FP32
+  // storage): p=1 8x4x4 425 / 271 -> 1x4x4 403 / 267 (2x4x4 406, 4x4x4
+  // 413, 2x8x4 416, 16x4x4 440), p=2 8x8x4 443 / 283 -> 2x8x4 431 / 278
+  // (4x8x4 434, 2x8x8 432, 4x4x4 454), and at p = 6: 64x8x1 375 / 248 ->
+  // 4x8x2 367 / 233 (8x8x1 371, 16x8x1 372); at p = 3..5 they were within
+  // 1 %.
   if (p <= 1) {
-    bx = 8; by = 4; bz = 4;
-  } else if (p == 2 || p == 4) {
+    bx = 1; by = 4; bz = 4;
+  } else if (p == 2) {
+    bx = 2; by = 8; bz = 4;
+  } else if (p == 4) {
     bx = 8; by = 8; bz = 4;
   } else if (p <= 5) {
     bx = 16; by = 8; bz = 2;
+  } else if (p == 6) {
+    bx = 4; by = 8; bz = 2;
   } else {
     bx = 64; by = 8; bz = 1;
   }
