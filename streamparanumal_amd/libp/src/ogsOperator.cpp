@@ -333,14 +333,15 @@ void HexGsBlockShape(const int p, int &bx, int &by, int &bz) {
   // 304; p = 7 and 15 take line tiles, HexGsLineRows).  Narrow blocks later did better at p = 1 and 2, whose tiles
   // then take the x points of several blocks (~1e8 nodes, FP64 / FP32
   // storage): p=1 8x4x4 425 / 271 -> 1x4x4 403 / 267 (2x4x4 406, 4x4x4
-  // 413, 2x8x4 416, 16x4x4 440), p=2 8x8x4 443 / 283 -> 2x8x4 431 / 278
-  // (4x8x4 434, 2x8x8 432, 4x4x4 454), and at p = 6: 64x8x1 375 / 248 ->
+  // 413, 2x8x4 416, 16x4x4 440), p=2 8x8x4 443 / 283 -> 2x8x6 425 / 280
+  // (2x8x4 433, 2x6x6 426, 2x10x6 427, 4x8x6 428, 2x8x8 432, 4x4x4 454),
+  // and at p = 6: 64x8x1 375 / 248 ->
   // 4x8x2 367 / 233 (8x8x1 371, 16x8x1 372); at p = 3..5 they were within
   // 1 %.
   if (p <= 1) {
     bx = 1; by = 4; bz = 4;
   } else if (p == 2) {
-    bx = 2; by = 8; bz = 4;
+    bx = 2; by = 8; bz = 6;
   } else if (p == 4) {
     bx = 8; by = 8; bz = 4;
   } else if (p <= 5) {
