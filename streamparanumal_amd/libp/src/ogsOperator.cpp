@@ -62,6 +62,12 @@ ogsOperator_t::ogsOperator_t(platform_t &platform_, Kind kind_, const dlong Nrow
     signedSrb = memberFlagN.length() > 0;
   }
   SetBlockNnz(blockNnz_);
+  // a small operator at the default capacity takes smaller tiles, so that
+  // its launches still fill the GPU: halved (down to 256 entries) while it
+  // would make fewer than 512 tiles.  Measured on MI355X, 8^3 p=7 (262,144
+  // entries), us per gather+scatter: 2048 6.46, 1024 5.09, 512 4.81, 256 4.85
+  if (gsBlockNnz_ == defaultGsBlockNnz)
+    while (gsBlockNnz_ > 256 && (int64_t)nnzT < 512 * (int64_t)gsBlockNnz_) gsBlockNnz_ /= 2;
   SetGsBlockNnz(gsBlockNnz_);
   auto localRows = [&](dlong nrows, const memory<dlong> &rs, const memory<dlong> &ci) {
     memory<dlong> m(Ncols > 0 ? Ncols : 1);
