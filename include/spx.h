@@ -256,7 +256,8 @@ typedef struct {
                               spx_ogs_gather_scatter_srb_signed with SPX_SYM,
                               spx_ogs_gather_scatter_srb_mixed) streams the values of
                               nodes of 2^e bytes (K values of the type, all K in one pass) past the
-                              caches with non-temporal loads and stores; the layout builders set
+                              caches with non-temporal loads and stores (256-lane tiles; the
+                              one-wave form keeps ordinary ones); the layout builders set
                               the bits for which whole 128-byte lines make up most of the tiles'
                               accesses.  0: ordinary loads and stores.  The other sorted-run
                               entry points ignore it. */
